@@ -1,0 +1,87 @@
+"""Build libfd2.so (HIP kernels + C host library) in-tree for gfx950.
+
+    python -m hybridmantis_amd.build          # or hybridmantis_amd.build.build()
+
+hipcc compiles the kernels and the host library into one shared object next
+to this file; nothing is installed elsewhere.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libfd2.so")
+
+HIP_SOURCES = ["fd2_kernels.hip", "fd2_host.cpp", "fd2_mlp.hip"]
+CXX_SOURCES = ["fd2_io.cpp", "fd2_fortran.cpp"]
+
+ARCH = os.environ.get("FD2_OFFLOAD_ARCH", "gfx950")
+
+# Packed-FP32 instructions are left to the compiler's own judgement off:
+# every HIP translation unit is built with the feature disabled.
+# -ffp-contract=off: no implicit FMA, so the transport's float expressions
+# round exactly like the CPU restatement of the same expressions.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+             "-I", os.path.join(ROOT, "include")]
+
+
+def _hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP toolchain is required to build libfd2")
+
+
+def build(verbose: bool = False, jobs: int = 4) -> str:
+    hipcc = _hipcc()
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cmds, objs = [], []
+    for src in HIP_SOURCES + CXX_SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        if src in HIP_SOURCES:
+            cmd = [hipcc, "-x", "hip", *HIP_FLAGS, "-c", path, "-o", obj]
+        else:
+            cmd = [hipcc, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I",
+                   os.path.join(ROOT, "include"), "-c", path, "-o", obj]
+        cmds.append(cmd)
+        objs.append(obj)
+    procs = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                      stderr=subprocess.STDOUT, text=True))
+        while len([p for p in procs if p.poll() is None]) >= jobs:
+            procs[0].wait()
+    failed = []
+    for cmd, p in zip(cmds, procs):
+        out, _ = p.communicate()
+        if out and (verbose or p.returncode):
+            sys.stderr.write(out)
+        if p.returncode:
+            failed.append(cmd[-3])
+    if failed:
+        raise RuntimeError(f"hipcc failed for {failed}")
+    tmp = LIB + ".tmp"
+    link = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    if verbose:
+        print(" ".join(link), flush=True)
+    subprocess.run(link, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,827 @@
+// fd2_device.h -- device-side optical-photon transport for gfx950.
+//
+// The per-photon physics of fastDETECT2 as the reference C path defines it
+// (main/kernel_cuda_c_ver1_0.cu, C branches), in FP32 with per-lane state in
+// registers.  The control flow (including every random draw, resample loop
+// and the stale-state quirks) follows the reference step for step because
+// the draw order decides each photon's fate; the arithmetic is plain FP32
+// (no double anywhere on the GPU).
+//
+// Geometry of the reference ("on-the-fly" columns): every column is an
+// infinite vertical cylinder of radius R; the first one is centred on the
+// event's (x,y); after crosstalk or a flight between columns the next column
+// is centred at pos - R*n for a sampled inward normal n.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fd2_math.h"
+
+namespace fd2 {
+
+// float forms of the reference's constants (main/kernel_cuda_c_ver1_0.cu:56-58)
+constexpr float kTwoPi = 6.283185308f;
+constexpr float kPi = 3.14159265f;
+// The reference compares floats against the double constant eps =
+// 8.1929093e-6 (and 1 +- eps, R - 1e-5, asin(nIC/nC)).  Each such test is
+// done here against the float threshold that gives the same answer for every
+// float operand: x < c  <=>  x < lt(c), x > c  <=>  x > gt(c), with lt(c) the
+// smallest float >= c and gt(c) the largest float <= c (host-computed, in
+// GeomDev).
+// Safety cap on one photon's loop passes (in-column steps plus the
+// boundary_analysis / top-surface loops the reference leaves unbounded); a
+// photon past it is ended as lost.  Typical photons need well under 10^3.
+constexpr int32_t kMaxLoops = 1 << 20;
+
+// Per-geometry transport constants, derived once on the host.
+struct GeomDev {
+  float xdet, ydet, H, R, n1, n2, top_absfrac, bulk, beta, d_min, d_max;
+  float lbx, lby, ubx, uby, sensor_refl;
+  float half_h;        // H/2
+  float r_nudge;       // gt(R - 1e-5): out-of-column threshold
+  float depth20;       // H*0.2 (crosstalk band)
+  float neg_inv_bulk;  // -1/bulk
+  float asin_n2n1;     // gt(asin(n2/n1)): TIR inside a column
+  float asin_n1n2;     // gt(asin(n1/n2)) (NaN if n1/n2 > 1): TIR from the gap
+  float ratio12, ratio21;  // n1/n2, n2/n1
+  float dspan;         // d_max - d_min
+  float eps_lt, eps_gt;    // x < eps, x > eps
+  float norm_lo, norm_hi;  // |v| < 1 - eps, |v| > 1 + eps
+  int32_t pixel, xdim, ydim, num_hist;
+  int32_t gap_fresnel; // (n1/n2) >= 1.57: gap side recomputes Pr/Pt
+  int32_t pad0;
+  unsigned long long* prf;   // xdim*ydim
+  uint32_t* hist;            // num_hist
+};
+
+// ------------------------------------------------------------------ RNG
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0,
+                                              uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+    uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Production stream: xoshiro128** (Blackman & Vigna 2018) per photon, its
+// 128-bit state = Philox4x32-10(counter (photon,0,0,0), key (seed, event)).
+struct CounterRng {
+  uint32_t s0, s1, s2, s3;
+  __device__ __forceinline__ void seed(uint32_t key0, uint32_t event,
+                                       uint32_t photon) {
+    uint32_t c[4] = {photon, 0u, 0u, 0u};
+    philox4x32_10(c, key0, event);
+    s0 = c[0]; s1 = c[1]; s2 = c[2]; s3 = c[3];
+    if ((s0 | s1 | s2 | s3) == 0u) s0 = 0x9E3779B9u;
+  }
+  __device__ __forceinline__ uint32_t next() {
+    uint32_t x = s1 * 5u;
+    uint32_t out = ((x << 7) | (x >> 25)) * 9u;
+    uint32_t t = s1 << 9;
+    s2 ^= s0;
+    s3 ^= s1;
+    s1 ^= s2;
+    s0 ^= s3;
+    s2 ^= t;
+    s3 = (s3 << 11) | (s3 >> 21);
+    return out;
+  }
+  // 24 random bits -> (0,1], exact in float
+  __device__ __forceinline__ float u() {
+    return (float)((next() >> 8) + 1u) * (1.0f / 16777216.0f);
+  }
+};
+
+// Trajectory-test stream: RANECU (L'Ecuyer 1988) exactly as
+// main/kernel_cuda_c_ver1_0.cu:3694-3711, float result in (0,1].
+struct RanecuRng {
+  int32_t s0, s1;
+  __device__ __forceinline__ float u() {
+    int32_t q1 = s0 / 53668;
+    s0 = 40014 * (s0 - q1 * 53668) - q1 * 12211;
+    int32_t q2 = s1 / 52774;
+    s1 = 40692 * (s1 - q2 * 52774) - q2 * 3791;
+    if (s0 < 0) s0 += 2147483563;
+    if (s1 < 0) s1 += 2147483399;
+    int32_t z = s0 - s1;
+    if (z < 1) z += 2147483562;
+    return (float)z * (float)(1.0 / 2147483563.0);
+  }
+};
+
+// a^e mod m with 64-bit products (the reference's jump-ahead,
+// main/kernel_cuda_c_ver1_0.cu:3516-3572)
+__device__ __forceinline__ uint32_t powmod_u32(uint32_t a, uint64_t e,
+                                               uint32_t m) {
+  uint64_t r = 1, b = a % m;
+  while (e) {
+    if (e & 1u) r = (r * b) % m;
+    b = (b * b) % m;
+    e >>= 1;
+  }
+  return (uint32_t)r;
+}
+
+__device__ __forceinline__ void ranecu_init(RanecuRng& g, int batch,
+                                            int32_t seed_input) {
+  uint64_t leap = (uint64_t)(batch + 1) * (uint64_t)(50000 * 1000);
+  uint64_t s = (uint64_t)(uint32_t)seed_input;
+  g.s0 = (int32_t)((s * powmod_u32(40014u, leap, 2147483563u)) % 2147483563u);
+  g.s1 = (int32_t)((s * powmod_u32(40692u, leap, 2147483399u)) % 2147483399u);
+}
+
+// ----------------------------------------------------------- tallies
+
+enum Fate : int {
+  kNone = 0, kDetect = 1, kDetectOut = 2, kAbsTop = 3, kAbsBulk = 4,
+  kLost = 5, kOutOfCol = 6, kTheta1 = 7
+};
+
+// photon state, all in registers
+struct Photon {
+  float px, py, pz;   // position
+  float ux, uy, uz;   // direction cosines
+  float nx, ny, nz;   // surface normal
+  float xc, yc;       // current column axis
+  float ex, ey, ez;   // event position
+  int32_t hist;       // 1-based history
+  bool first;         // no boundary interaction yet (num_rebound == 0)
+  int32_t loops;      // loop passes so far (kMaxLoops cap)
+  int fate;
+  int32_t pix;
+  float fx, fy;
+};
+
+// Detection: pixel floor((x-lb)/pitch), inside iff lb <= x <= ub
+// (main/kernel_cuda_c_ver1_0.cu:3402-3419); per-history count at hist-1.
+__device__ __forceinline__ void tally_detect(Photon& p, const GeomDev& g) {
+  p.fate = kDetectOut;
+  p.pix = -1;
+  p.fx = p.px;
+  p.fy = p.py;
+  int i = (int)floorf((p.px - g.lbx) / (float)g.pixel);
+  int j = (int)floorf((p.py - g.lby) / (float)g.pixel);
+  if (p.px <= g.ubx && p.py <= g.uby && p.px >= g.lbx && p.py >= g.lby &&
+      i >= 0 && i < g.xdim && j >= 0 && j < g.ydim) {
+    int idx = i * g.ydim + j;
+    atomicAdd(&g.prf[idx], 1ull);
+    p.fate = kDetect;
+    p.pix = idx;
+  }
+  if (p.hist >= 1 && p.hist <= g.num_hist) atomicAdd(&g.hist[p.hist - 1], 1u);
+}
+
+// ------------------------------------------------------- vector helpers
+
+__device__ __forceinline__ void renorm(float& x, float& y, float& z,
+                                       const GeomDev& g) {
+  float nrm = sqrtf(x * x + y * y + z * z);
+  if (nrm < g.norm_lo || nrm > g.norm_hi) {
+    x = x / nrm;
+    y = y / nrm;
+    z = z / nrm;
+  }
+}
+
+// trans_dir_cos (main/kernel_cuda_c_ver1_0.cu:3169-3205)
+__device__ __forceinline__ void reflect(Photon& p, const GeomDev& g) {
+  float ca = -p.ux * p.nx + -p.uy * p.ny + -p.uz * p.nz;
+  p.ux = 2.0f * ca * p.nx + p.ux;
+  p.uy = 2.0f * ca * p.ny + p.uy;
+  p.uz = 2.0f * ca * p.nz + p.uz;
+  renorm(p.ux, p.uy, p.uz, g);
+}
+
+__device__ __forceinline__ void refract(Photon& p, float t1, float t2,
+                                        const GeomDev& g) {
+  float ca = -p.ux * p.nx + -p.uy * p.ny + -p.uz * p.nz;
+  float s2, c2;
+  fd2_sincos(t2, &s2, &c2);
+  float ratio = s2 / fd2_sin(t1);
+  p.ux = -p.nx * c2 - ratio * (p.ux + ca * p.nx);
+  p.uy = -p.ny * c2 - ratio * (p.uy + ca * p.ny);
+  p.uz = -p.nz * c2 - ratio * (p.uz + ca * p.nz);
+  renorm(p.ux, p.uy, p.uz, g);
+}
+
+// RoughSurface (main/kernel_cuda_c_ver1_0.cu:3261-3300): perturbation with
+// z = 2s-1 and radius sqrt(1-s^2) (s, not z), normalised, then
+// n <- normalise(beta*p + n).
+template <class Rng>
+__device__ __forceinline__ void rough(Rng& rng, float& nx, float& ny, float& nz,
+                                      float beta) {
+  float s = rng.u();
+  float qz = 2.0f * s - 1.0f;
+  float rr = sqrtf(1.0f - s * s);
+  s = rng.u();
+  float th = s * 2.0f * kPi;
+  float sn, cs;
+  fd2_sincos(th, &sn, &cs);
+  float qx = rr * cs, qy = rr * sn;
+  float nb = sqrtf(qx * qx + qy * qy + qz * qz);
+  qx = qx / nb;
+  qy = qy / nb;
+  qz = qz / nb;
+  float rx = beta * qx + nx, ry = beta * qy + ny, rz = beta * qz + nz;
+  nb = sqrtf(rx * rx + ry * ry + rz * rz);
+  nx = rx / nb;
+  ny = ry / nb;
+  nz = rz / nb;
+}
+
+// isotropic direction at photon termination / trapped restart
+// (main/kernel_cuda_c_ver1_0.cu:578-592)
+template <class Rng>
+__device__ __forceinline__ void emit_dir(Rng& rng, Photon& p, const GeomDev& g) {
+  float r = rng.u() * 2.0f - 1.0f;
+  p.uz = r;
+  float rr = sqrtf(1.0f - r * r);
+  float th = rng.u() * kTwoPi;
+  float sn, cs;
+  fd2_sincos(th, &sn, &cs);
+  p.ux = rr * cs;
+  p.uy = rr * sn;
+  renorm(p.ux, p.uy, p.uz, g);
+}
+
+// first photon of an event: |2u-1| > 0.01 by rejection
+// (main/hybridMANTIS_c_ver1_0.c:257-277)
+template <class Rng>
+__device__ __forceinline__ void emit_first_dir(Rng& rng, Photon& p,
+                                               const GeomDev& g) {
+  float r = rng.u() * 2.0f - 1.0f;
+  while (fabsf(r) <= 0.01f) r = rng.u() * 2.0f - 1.0f;
+  p.uz = r;
+  float rr = sqrtf(1.0f - r * r);
+  float th = rng.u() * kTwoPi;
+  float sn, cs;
+  fd2_sincos(th, &sn, &cs);
+  p.ux = rr * cs;
+  p.uy = rr * sn;
+  renorm(p.ux, p.uy, p.uz, g);
+}
+
+// ---------------------------------------------------------- transport
+
+__device__ __forceinline__ bool out_of_slab(const Photon& p, const GeomDev& g) {
+  return p.px < g.eps_lt || p.px > g.xdet || p.py < g.eps_lt || p.py > g.ydet ||
+         p.pz < -g.half_h || p.pz > g.half_h;
+}
+
+
+// Optional per-step trace (FP64 re-trace check): 14 floats per step.
+struct StepSink {
+  float* buf;
+  int32_t* count;
+  int32_t max_steps;
+  int32_t photon;
+  bool on;
+};
+
+// dist_to_surface (main/kernel_cuda_c_ver1_0.cu:966-1185): move to the
+// nearer of the column wall (larger ray/cylinder root) and the plane
+// z = +-H/2; lost check; out-of-column nudge (back 0.5 um along -d, halving
+// on overshoot, at most 10+10 tries).  Returns the distance, or -99 when the
+// photon was lost or left its column (fate set).
+template <bool TRACE>
+__device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
+                                                 StepSink* sink) {
+  const float H2 = g.half_h, R = g.R;
+  const float t0 = p.px, t1 = p.py, t2 = p.pz;
+  if (p.first) {
+    p.xc = p.ex;
+    p.yc = p.ey;
+  }
+  const float xc = p.xc, yc = p.yc;
+  const float dx0 = p.px - xc, dy0 = p.py - yc;
+  const float qa = p.ux * p.ux + p.uy * p.uy;
+  const float qb = 2.0f * (dx0 * p.ux + dy0 * p.uy);
+  const float qc = dx0 * dx0 + dy0 * dy0 - R * R;
+  const float sq = sqrtf(qb * qb - 4.0f * qa * qc);
+  const float r1 = (-qb + sq) / (2.0f * qa);
+  const float r2 = (-qb - sq) / (2.0f * qa);
+  float dcyl = 999.0f;
+  if (r1 >= r2)
+    dcyl = r1;
+  else if (r2 > r1)
+    dcyl = r2;
+  float d;
+  if (p.uz > 0.0f) {
+    const float dpl = (H2 - p.pz) / p.uz;
+    if (fabsf(p.ux) < g.eps_lt && fabsf(p.uy) < g.eps_lt && fabsf(p.uz - 1.0f) < g.eps_lt) {
+      d = dpl;
+      p.pz = H2;
+    } else if (dpl >= dcyl) {
+      d = dcyl;
+      p.pz = t2 + d * p.uz;
+    } else {
+      d = dpl;
+      p.pz = H2;
+    }
+  } else if (p.uz < 0.0f) {
+    const float dpl = (-H2 - p.pz) / p.uz;
+    if (fabsf(p.ux) < g.eps_lt && fabsf(p.uy) < g.eps_lt && fabsf(p.uz + 1.0f) < g.eps_lt) {
+      d = dpl;
+      p.pz = -H2;
+    } else if (dpl >= dcyl) {
+      d = dcyl;
+      p.pz = t2 + d * p.uz;
+    } else {
+      d = dpl;
+      p.pz = -H2;
+    }
+  } else {
+    d = dcyl;
+    p.pz = t2 + d * p.uz;
+  }
+  p.px = t0 + d * p.ux;
+  p.py = t1 + d * p.uy;
+
+  int code = 0;
+  if (out_of_slab(p, g)) {
+    code = 1;
+  } else {
+    float dx = p.px - xc, dy = p.py - yc;
+    float rn = sqrtf(dx * dx + dy * dy);
+    int moved = 0, shrunk = 0;
+    float step = 0.5f;
+    while (rn > g.r_nudge && moved < 10 && shrunk < 10) {
+      const float cx = p.px + step * (-p.ux), cy = p.py + step * (-p.uy);
+      const float ex = cx - xc, ey = cy - yc;
+      const float rc = sqrtf(ex * ex + ey * ey);
+      if (rc > rn) {
+        step = step / 2.0f;
+        shrunk++;
+      } else {
+        p.px = cx;
+        p.py = cy;
+        rn = rc;
+        moved++;
+      }
+    }
+    if (rn > g.r_nudge)
+      code = 2;
+    else if (out_of_slab(p, g))
+      code = 1;
+  }
+  if (TRACE && sink->on) {
+    const int slot = atomicAdd(sink->count, 1);
+    if (slot < sink->max_steps) {
+      float* o = sink->buf + (size_t)slot * 14;
+      o[0] = t0; o[1] = t1; o[2] = t2;
+      o[3] = p.ux; o[4] = p.uy; o[5] = p.uz;
+      o[6] = xc; o[7] = yc;
+      o[8] = p.px; o[9] = p.py; o[10] = p.pz;
+      o[11] = __int_as_float(code);
+      o[12] = __int_as_float(sink->photon);
+      o[13] = 0.0f;
+    }
+  }
+  if (code == 1) {
+    p.fate = kLost;
+    return -99.0f;
+  }
+  if (code == 2) {
+    p.fate = kOutOfCol;
+    return -99.0f;
+  }
+  return d;
+}
+
+// isotropic (main/kernel_cuda_c_ver1_0.cu:702-732): in-column step, then
+// bulk absorption at -ln(u)/mu (drawn even when the step was lost).
+// true = photon ended.
+template <class Rng, bool TRACE>
+__device__ __forceinline__ bool isotropic(Rng& rng, Photon& p, const GeomDev& g,
+                                          StepSink* sink) {
+  const float ds = dist_to_surface<TRACE>(p, g, sink);
+  float da = 999999.0f;
+  if (g.bulk > 0.0f) da = g.neg_inv_bulk * fd2_log(rng.u());
+  if (fabsf(ds - (-99.0f)) < g.eps_lt) return true;
+  if (ds < da && ds >= 0.0f) return false;
+  if (ds >= da && ds >= 0.0f) {
+    p.fate = kAbsBulk;
+    return true;
+  }
+  return false;
+}
+
+// detection (main/kernel_cuda_c_ver1_0.cu:3372-3427): on the sensor plane,
+// specular reflection with probability sensorRefl (then another in-column
+// step), else detected.  true = photon ended.
+template <class Rng, bool TRACE>
+__device__ __forceinline__ bool detection(Rng& rng, Photon& p, const GeomDev& g,
+                                          StepSink* sink) {
+  if (fabsf(p.pz - (-g.half_h)) < g.eps_lt) {
+    if (rng.u() < g.sensor_refl) {
+      p.nx = 0.0f;
+      p.ny = 0.0f;
+      p.nz = 1.0f;
+      reflect(p, g);
+      return isotropic<Rng, TRACE>(rng, p, g, sink);
+    }
+    tally_detect(p, g);
+    return true;
+  }
+  return false;
+}
+
+// New column orientation (transmit/boundary_analysis/refl_bottom,
+// main/kernel_cuda_c_ver1_0.cu:2452-2488, 2501-2538, 1769-1804, 3036-3070):
+// n = -d perturbed with beta = 1, projected on xy and renormalised, then
+// perturbed with beta (always for refl_bottom); accept when
+// acos(-d.n) <= 1.57, else resample while the caller's counter < 100.
+// true = photon killed (theta1).
+template <class Rng>
+__device__ __forceinline__ bool column_normal(Rng& rng, Photon& p,
+                                              const GeomDev& g, float mx,
+                                              float my, float mz, int& tries,
+                                              bool always_rough) {
+  for (;;) {
+    float nx = mx, ny = my, nz = mz;
+    rough(rng, nx, ny, nz, 1.0f);
+    const float den = sqrtf(nx * nx + ny * ny);
+    nz = 0.0f;
+    nx = nx / den;
+    ny = ny / den;
+    if (always_rough || g.beta > 0.0f) rough(rng, nx, ny, nz, g.beta);
+    p.nx = nx;
+    p.ny = ny;
+    p.nz = nz;
+    const float ang = fd2_acos(mx * nx + my * ny + mz * nz);
+    if (ang < 0.0f || ang > 1.57f) {
+      if (tries < 100) {
+        tries++;
+        continue;
+      }
+      p.fate = kTheta1;
+      return true;
+    }
+    return false;
+  }
+}
+
+__device__ __forceinline__ bool at_top_up(const Photon& p, const GeomDev& g) {
+  return fabsf(p.pz - g.half_h) < g.eps_lt && p.uz > 0.0f;
+}
+
+__device__ __forceinline__ bool outside_xy(const Photon& p, const GeomDev& g) {
+  return p.px < g.eps_lt || p.px > g.xdet || p.py < g.eps_lt || p.py > g.ydet;
+}
+
+// refl_bottom (main/kernel_cuda_c_ver1_0.cu:2985-3087).  true = ended.
+template <class Rng>
+__device__ __forceinline__ bool refl_bottom(Rng& rng, Photon& p,
+                                            const GeomDev& g) {
+  const float tx = p.px, ty = p.py, tz = p.pz;
+  const float mx = -p.ux, my = -p.uy, mz = -p.uz;
+  const float d = rng.u() * g.dspan + g.d_min;
+  const float dtop = (g.half_h - tz) / p.uz;
+  p.px = tx + p.ux * d;
+  p.py = ty + p.uy * d;
+  p.pz = tz + p.uz * d;
+  if (outside_xy(p, g)) {
+    p.fate = kLost;
+    return true;
+  }
+  if (p.pz > g.half_h) {
+    if (dtop < d && dtop > g.eps_gt) {
+      p.px = tx + p.ux * dtop;
+      p.py = ty + p.uy * dtop;
+      p.pz = g.half_h;
+      return false;
+    }
+    p.fate = kLost;
+    return true;
+  }
+  int tries = 0;
+  return column_normal(rng, p, g, mx, my, mz, tries, true);
+}
+
+// sensor plane reached between columns: reflect + refl_bottom, or detect.
+template <class Rng>
+__device__ __forceinline__ bool gap_sensor(Rng& rng, Photon& p,
+                                           const GeomDev& g) {
+  if (rng.u() < g.sensor_refl) {
+    p.nx = 0.0f;
+    p.ny = 0.0f;
+    p.nz = 1.0f;
+    reflect(p, g);
+    return refl_bottom(rng, p, g);
+  }
+  tally_detect(p, g);
+  return true;
+}
+
+// transmit with flagCCT = 0 (main/kernel_cuda_c_ver1_0.cu:2421-2865):
+// flight of U(dmin,dmax) between columns against the detector's top
+// (reflector/absorber), bottom (sensor) and sides, then the next column's
+// orientation.  true = ended.
+template <class Rng>
+__device__ __forceinline__ bool transmit_gap(Rng& rng, Photon& p,
+                                          const GeomDev& g) {
+  const float H2 = g.half_h;
+  float tx = p.px, ty = p.py, tz = p.pz;
+  float mx = -p.ux, my = -p.uy, mz = -p.uz;
+  int tries1 = 0, tries2 = 0;
+  float d = rng.u() * g.dspan + g.d_min;
+  p.px = tx + p.ux * d;
+  p.py = ty + p.uy * d;
+  p.pz = tz + p.uz * d;
+  float dtop = (H2 - tz) / p.uz;
+  float dbot = (-H2 - tz) / p.uz;
+  if (outside_xy(p, g)) {
+    p.fate = kLost;
+    return true;
+  }
+  if (p.pz < -H2 || p.pz > H2) {
+    if (dtop < d && dtop > g.eps_gt) {
+      p.px = tx + p.ux * dtop;
+      p.py = ty + p.uy * dtop;
+      p.pz = H2;
+    } else if (dbot < d && dbot > g.eps_gt) {
+      p.px = tx + p.ux * dbot;
+      p.py = ty + p.uy * dbot;
+      p.pz = -H2;
+      if (gap_sensor(rng, p, g)) return true;
+      if (at_top_up(p, g)) goto top;
+    } else {
+      p.fate = kLost;
+      return true;
+    }
+  }
+  if (column_normal(rng, p, g, mx, my, mz, tries1, false)) return true;
+  if (fabsf(p.px) < g.eps_lt || fabsf(p.px - g.xdet) < g.eps_lt ||
+      fabsf(p.py) < g.eps_lt || fabsf(p.py - g.ydet) < g.eps_lt) {
+    p.fate = kLost;
+    return true;
+  }
+top:
+  if (++p.loops > kMaxLoops) {
+    p.fate = kLost;
+    return true;
+  }
+  if (at_top_up(p, g)) {
+    p.nx = 0.0f;
+    p.ny = 0.0f;
+    p.nz = -1.0f;
+    if (g.top_absfrac > 0.0f && rng.u() < g.top_absfrac) {
+      p.fate = kAbsTop;
+      return true;
+    }
+    p.uz = -fabsf(rng.u() * 2.0f - 1.0f);
+    {
+      const float rr = sqrtf(1.0f - p.uz * p.uz);
+      const float th = rng.u() * kTwoPi;
+      float sn, cs;
+      fd2_sincos(th, &sn, &cs);
+      p.ux = rr * cs;
+      p.uy = rr * sn;
+    }
+    tx = p.px;
+    ty = p.py;
+    tz = p.pz;
+    mx = -p.ux;
+    my = -p.uy;
+    mz = -p.uz;
+    d = rng.u() * g.dspan + g.d_min;
+    dbot = (-H2 - tz) / p.uz;
+    p.px = tx + p.ux * d;
+    p.py = ty + p.uy * d;
+    p.pz = tz + p.uz * d;
+    if (outside_xy(p, g)) {
+      p.fate = kLost;
+      return true;
+    }
+    if (p.pz < -H2 || p.pz > H2) {
+      if (dbot < d && dbot > g.eps_gt) {
+        p.px = tx + p.ux * dbot;
+        p.py = ty + p.uy * dbot;
+        p.pz = -H2;
+        if (gap_sensor(rng, p, g)) return true;
+        if (at_top_up(p, g)) goto top;
+      } else {
+        p.fate = kLost;
+        return true;
+      }
+    }
+    if (column_normal(rng, p, g, mx, my, mz, tries2, false)) return true;
+  }
+  if (fabsf(p.pz - (-H2)) < g.eps_lt) {
+    if (gap_sensor(rng, p, g)) return true;
+    if (at_top_up(p, g)) goto top;
+  }
+  return false;
+}
+
+// Fresnel probabilities (main/kernel_cuda_c_ver1_0.cu:1811-1871) in FP32:
+// Pr = (tan^2(t1-t2)/tan^2(t1+t2) + sin^2(t1-t2)/sin^2(t1+t2)) / 2,
+// Pt = (sin2t1 sin2t2/(sin^2(t1+t2)cos^2(t1-t2)) + sin2t1 sin2t2/sin^2(t1+t2)) / 2
+__device__ __forceinline__ void fresnel_pq(float t1, float t2, float& pr,
+                                           float& pt) {
+  float sm, cm, sp, cp;
+  fd2_sincos(t1 - t2, &sm, &cm);
+  fd2_sincos(t1 + t2, &sp, &cp);
+  const float tm = sm / cm, tp = sp / cp;
+  const float sp2 = sp * sp;
+  pr = 0.5f * ((tm * tm) / (tp * tp) + (sm * sm) / sp2);
+  const float s12 = fd2_sin(2.0f * t1) * fd2_sin(2.0f * t2);
+  pt = 0.5f * (s12 / (sp2 * (cm * cm)) + s12 / sp2);
+}
+
+// boundary_analysis (main/kernel_cuda_c_ver1_0.cu:1582-1958).  true = ended.
+template <class Rng>
+__device__ __forceinline__ bool boundary_analysis(Rng& rng, Photon& p,
+                                               const GeomDev& g) {
+  const float H2 = g.half_h, R = g.R;
+  // top reflector reached inside a column (K:1610-1634)
+  if (fabsf(p.pz - H2) < g.eps_lt && p.uz > 0.0f) {
+    if (g.top_absfrac > 0.0f && rng.u() < g.top_absfrac) {
+      p.fate = kAbsTop;
+      return true;
+    }
+    p.nx = 0.0f;
+    p.ny = 0.0f;
+    p.nz = -1.0f;
+    p.uz = -fabsf(rng.u() * 2.0f - 1.0f);
+    const float rr = sqrtf(1.0f - p.uz * p.uz);
+    const float th = rng.u() * kTwoPi;
+    float sn, cs;
+    fd2_sincos(th, &sn, &cs);
+    p.ux = rr * cs;
+    p.uy = rr * sn;
+    return false;
+  }
+  // crosstalk probability, piecewise linear in depth (K:1639-1652)
+  float cct = 0.0f;
+  if (p.pz <= H2 && p.pz >= H2 - g.depth20)
+    cct = 1.0f;
+  else if (p.pz < H2 - g.depth20 && p.pz >= 0.0f)
+    cct = p.pz / (2.0f * (H2 - g.depth20)) + 0.5f;
+  else if (p.pz < 0.0f && p.pz >= -H2)
+    cct = (p.pz - (-H2)) / (2.0f * (-H2)) + 1.0f;
+  if (rng.u() < cct) {
+    // neighbour column with a random orientation, direction kept (K:2452-2488)
+    int tries = 0;
+    if (column_normal(rng, p, g, -p.ux, -p.uy, -p.uz, tries, false)) return true;
+    p.xc = p.px + R * (-p.nx);
+    p.yc = p.py + R * (-p.ny);
+    return false;
+  }
+
+  bool inside = true;  // flag_call_transmit
+  int theta_tries = 0, smooth_tries = 0, reperturb = 0, normal_tries = 0;
+  float pr = 0.0f, pt = 0.0f, t1 = 0.0f, t2 = 0.0f;
+  float onx, ony, onz, odx, ody, odz;
+  float mx, my, mz;
+
+prpt:
+  if (++p.loops > kMaxLoops) {
+    p.fate = kLost;
+    return true;
+  }
+  if (inside) {
+    const float ax = p.xc - p.px, ay = p.yc - p.py;
+    const float mag = sqrtf(ax * ax + ay * ay);
+    p.nx = ax / mag;
+    p.ny = ay / mag;
+    p.nz = 0.0f;
+    if (g.beta > 0.0f) rough(rng, p.nx, p.ny, p.nz, g.beta);
+  } else {
+    p.xc = p.px + R * (-p.nx);
+    p.yc = p.py + R * (-p.ny);
+  }
+  onx = p.nx; ony = p.ny; onz = p.nz;
+  odx = p.ux; ody = p.uy; odz = p.uz;
+
+reperturb_point:
+  p.nx = onx; p.ny = ony; p.nz = onz;
+  p.ux = odx; p.uy = ody; p.uz = odz;
+  mx = -p.ux; my = -p.uy; mz = -p.uz;
+  if (reperturb != 0) {
+    if (g.beta > 0.0f) rough(rng, p.nx, p.ny, p.nz, g.beta);
+    if (!inside) {
+      p.xc = p.px + R * (-p.nx);
+      p.yc = p.py + R * (-p.ny);
+    }
+  }
+
+no_perturbation:
+  t1 = mx * p.nx + my * p.ny + mz * p.nz;
+  if (t1 > 1.0f || t1 < 0.0f) {
+    if (!inside) {
+      if (column_normal(rng, p, g, mx, my, mz, normal_tries, false)) return true;
+    }
+    if (theta_tries < 100) {
+      theta_tries++;
+      goto prpt;
+    }
+    p.fate = kTheta1;
+    return true;
+  }
+  t1 = fd2_acos(t1);
+
+  if (inside) {
+    if (t1 > g.asin_n2n1) {
+      pr = 1.0f;
+      pt = 0.0f;
+    } else {
+      if (t1 < g.eps_lt) t1 = 0.00042f;
+      t2 = fd2_asin(g.ratio12 * fd2_sin(t1));
+      fresnel_pq(t1, t2, pr, pt);
+    }
+  } else if (!g.gap_fresnel) {
+    if (t1 > g.asin_n1n2) {
+      pr = 1.0f;
+      pt = 0.0f;
+    }
+  } else if (t1 < g.eps_lt) {
+    t1 = 0.00042f;
+    t2 = fd2_asin(g.ratio12 * fd2_sin(t1));
+    fresnel_pq(t1, t2, pr, pt);
+  } else {
+    t2 = fd2_asin(g.ratio21 * fd2_sin(t1));
+    fresnel_pq(t1, t2, pr, pt);
+  }
+  {
+    const float tot = pr + pt;
+    pr = pr / tot;
+    pt = pt / tot;
+  }
+
+  if (rng.u() < pr) {
+    reflect(p, g);
+    const float ang = fd2_acos(onx * p.ux + ony * p.uy + onz * p.uz);
+    if (ang > 1.57f) {
+      reperturb++;
+      if (reperturb < 4) goto reperturb_point;
+      p.nx = onx; p.ny = ony; p.nz = onz;
+      p.ux = odx; p.uy = ody; p.uz = odz;
+      mx = -p.ux; my = -p.uy; mz = -p.uz;
+      reperturb = 0;
+      if (smooth_tries < 25) {
+        smooth_tries++;
+        goto no_perturbation;
+      }
+      p.fate = kTheta1;
+      return true;
+    }
+    if (!inside) {
+      if (transmit_gap(rng, p, g)) return true;
+      goto prpt;
+    }
+    return false;
+  }
+  refract(p, t1, t2, g);
+  if (inside) {
+    inside = false;
+    if (transmit_gap(rng, p, g)) return true;
+    goto prpt;
+  }
+  return false;
+}
+
+// One algo() pass of the reference (main/kernel_cuda_c_ver1_0.cu:553-655)
+// fused with the boundary interaction that follows it: in-column step and
+// sensor check, then (if alive and not trapped) boundary_analysis.
+enum StepResult : int { kAlive = 0, kEnded = 1, kTrapped = 2 };
+
+template <class Rng, bool TRACE>
+__device__ __forceinline__ int photon_step(Rng& rng, Photon& p,
+                                           const GeomDev& g, StepSink* sink) {
+  if (++p.loops > kMaxLoops) {
+    p.fate = kLost;
+    return kEnded;
+  }
+  if (isotropic<Rng, TRACE>(rng, p, g, sink)) return kEnded;
+  if (detection<Rng, TRACE>(rng, p, g, sink)) return kEnded;
+  if (fabsf(p.uz) < g.eps_lt) return kTrapped;
+  p.first = false;
+  if (boundary_analysis(rng, p, g)) return kEnded;
+  return kAlive;
+}
+
+// trapped photon (|dcos.z| < eps): restart from the event with a new
+// direction; the column centre and rebound count are kept (C branch,
+// main/kernel_cuda_c_ver1_0.cu:609-644).  The normal drawn there is never
+// read before being overwritten, so only its two draws are consumed.
+template <class Rng>
+__device__ __forceinline__ void restart_trapped(Rng& rng, Photon& p,
+                                                const GeomDev& g) {
+  emit_dir(rng, p, g);
+  p.px = p.ex;
+  p.py = p.ey;
+  p.pz = p.ez;
+  if (g.beta > 0.0f) {
+    (void)rng.u();
+    (void)rng.u();
+  }
+}
+
+}  // namespace fd2

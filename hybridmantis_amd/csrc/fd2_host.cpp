@@ -1,0 +1,638 @@
+// fd2_host.cpp -- host side of libfd2: context, event staging, device work
+// queue, kernel launches and tallies.  Replaces the state machine of
+// gpuoptical_ (main/hybridMANTIS_cuda_ver1_0.cu:107-447): device memory is
+// owned by a context instead of being stashed in COMMON /gpumemaddr/, and
+// errors come back as status codes instead of aborting.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "fd2_internal.h"
+
+using namespace fd2;
+
+struct fd2_ctx {
+  int device = 0;
+  int n_cu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  bool timed = false;
+
+  int n_geoms = 0;
+  fd2_params params[FD2_MAX_GEOMS];
+  GeomDev hgeom[FD2_MAX_GEOMS];
+  GeomDev* d_geoms = nullptr;
+  unsigned long long* d_prf[FD2_MAX_GEOMS] = {};
+  uint32_t* d_hist[FD2_MAX_GEOMS] = {};
+  unsigned long long* d_counters = nullptr;
+
+  // staging and queue buffers, grown on demand
+  fd2_event* h_stage = nullptr;   // pinned
+  size_t cap_stage = 0;
+  fd2_event* d_events = nullptr;
+  size_t cap_events = 0;
+  uint32_t* d_chunk_off = nullptr;
+  unsigned long long* d_photon_off = nullptr;
+  size_t cap_off = 0;
+  uint32_t* d_block_chunks = nullptr;
+  unsigned long long* d_block_photons = nullptr;
+  size_t cap_blocks = 0;
+  uint32_t* d_cursor = nullptr;
+  uint32_t* d_error = nullptr;
+
+  uint32_t seed = 271828182u;
+  int64_t next_event = 0;
+  fd2_tuning tune{};
+};
+
+namespace {
+
+#define FD2_HIP(call)                      \
+  do {                                     \
+    hipError_t e_ = (call);                \
+    if (e_ != hipSuccess) return FD2_E_DEVICE; \
+  } while (0)
+
+int ilog2(int v) {
+  int l = 0;
+  while ((1 << (l + 1)) <= v) ++l;
+  return l;
+}
+
+int prf_side(double lb, double ub, int pixel) {
+  // xdim = ceil((ub - lb)/pixel) in float, as the reference drivers compute it
+  // (main/hybridMANTIS_c_ver1_0.c:163-164)
+  const float w = (float)ub - (float)lb;
+  return (int)std::ceil(w / (float)pixel);
+}
+
+int check_params(const fd2_params& p) {
+  if (!(p.detheight > 0) || !(p.detradius > 0) || !(p.detx > 0) || !(p.dety > 0))
+    return FD2_E_INVAL;
+  if (!(p.detnC > 0) || !(p.detnIC > 0) || p.detpixel <= 0) return FD2_E_INVAL;
+  if (!(p.detdmax >= p.detdmin)) return FD2_E_INVAL;
+  if (p.mynumhist < 0) return FD2_E_INVAL;
+  const int xd = prf_side(p.detlboundx, p.detuboundx, p.detpixel);
+  const int yd = prf_side(p.detlboundy, p.detuboundy, p.detpixel);
+  if (xd <= 0 || yd <= 0 || xd > FD2_MAX_PRF_SIDE || yd > FD2_MAX_PRF_SIDE)
+    return FD2_E_INVAL;
+  return FD2_OK;
+}
+
+// smallest float >= c, largest float <= c
+float float_ge(double c) {
+  float f = (float)c;
+  if ((double)f < c) f = std::nextafter(f, INFINITY);
+  return f;
+}
+float float_le(double c) {
+  float f = (float)c;
+  if ((double)f > c) f = std::nextafter(f, -INFINITY);
+  return f;
+}
+
+void derive(const fd2_params& p, GeomDev& g) {
+  std::memset(&g, 0, sizeof g);
+  g.xdet = (float)p.detx;
+  g.ydet = (float)p.dety;
+  g.H = (float)p.detheight;
+  g.R = (float)p.detradius;
+  g.n1 = (float)p.detnC;
+  g.n2 = (float)p.detnIC;
+  g.top_absfrac = (float)p.dettop;
+  g.bulk = (float)p.detbulk;
+  g.beta = (float)p.detbeta;
+  g.d_min = (float)p.detdmin;
+  g.d_max = (float)p.detdmax;
+  g.lbx = (float)p.detlboundx;
+  g.lby = (float)p.detlboundy;
+  g.ubx = (float)p.detuboundx;
+  g.uby = (float)p.detuboundy;
+  g.sensor_refl = (float)p.detsensorRefl;
+  g.half_h = g.H / 2.0f;
+  g.depth20 = g.H * 0.2f;
+  g.neg_inv_bulk = g.bulk > 0.0f ? -1.0f / g.bulk : 0.0f;
+  g.ratio12 = g.n1 / g.n2;
+  g.ratio21 = g.n2 / g.n1;
+  g.gap_fresnel = (g.ratio12 < 1.57f) ? 0 : 1;
+  g.dspan = g.d_max - g.d_min;
+  // exact float thresholds for the reference's double comparisons
+  const double eps = 8.1929093e-6;
+  g.eps_lt = float_ge(eps);
+  g.eps_gt = float_le(eps);
+  g.norm_lo = float_ge(1.0 - eps);
+  g.norm_hi = float_le(1.0 + eps);
+  g.r_nudge = float_le((double)g.R - 1e-5);
+  g.asin_n2n1 = float_le(std::asin((double)g.ratio21));
+  g.asin_n1n2 = g.ratio12 <= 1.0f ? float_le(std::asin((double)g.ratio12)) : NAN;
+  g.pixel = p.detpixel;
+  g.xdim = prf_side(p.detlboundx, p.detuboundx, p.detpixel);
+  g.ydim = prf_side(p.detlboundy, p.detuboundy, p.detpixel);
+  g.num_hist = p.mynumhist;
+}
+
+int grow(fd2_ctx* c, size_t n_events) {
+  if (n_events > c->cap_events) {
+    size_t cap = n_events + n_events / 4 + 1024;
+    if (c->d_events) (void)hipFree(c->d_events);
+    c->d_events = nullptr;
+    c->cap_events = 0;
+    FD2_HIP(hipMalloc(&c->d_events, cap * sizeof(fd2_event)));
+    c->cap_events = cap;
+  }
+  if (n_events + 1 > c->cap_off) {
+    size_t cap = n_events + n_events / 4 + 1025;
+    if (c->d_chunk_off) (void)hipFree(c->d_chunk_off);
+    if (c->d_photon_off) (void)hipFree(c->d_photon_off);
+    c->d_chunk_off = nullptr;
+    c->d_photon_off = nullptr;
+    c->cap_off = 0;
+    FD2_HIP(hipMalloc(&c->d_chunk_off, cap * sizeof(uint32_t)));
+    FD2_HIP(hipMalloc(&c->d_photon_off, cap * sizeof(unsigned long long)));
+    c->cap_off = cap;
+  }
+  const size_t nb = (n_events + kScanTile - 1) / kScanTile + 1;
+  if (nb > c->cap_blocks) {
+    size_t cap = nb + 64;
+    if (c->d_block_chunks) (void)hipFree(c->d_block_chunks);
+    if (c->d_block_photons) (void)hipFree(c->d_block_photons);
+    c->d_block_chunks = nullptr;
+    c->d_block_photons = nullptr;
+    c->cap_blocks = 0;
+    FD2_HIP(hipMalloc(&c->d_block_chunks, cap * sizeof(uint32_t)));
+    FD2_HIP(hipMalloc(&c->d_block_photons, cap * sizeof(unsigned long long)));
+    c->cap_blocks = cap;
+  }
+  return FD2_OK;
+}
+
+int grow_stage(fd2_ctx* c, size_t n_events) {
+  if (n_events <= c->cap_stage) return FD2_OK;
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  c->h_stage = nullptr;
+  c->cap_stage = 0;
+  const size_t cap = n_events + n_events / 4 + 1024;
+  FD2_HIP(hipHostMalloc(&c->h_stage, cap * sizeof(fd2_event), hipHostMallocDefault));
+  c->cap_stage = cap;
+  return FD2_OK;
+}
+
+int validate_host(const fd2_ctx* c, const fd2_event* ev, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (ev[i].N > FD2_MAX_PHOTONS_PER_EVENT) return FD2_E_TOO_MANY;
+    if (ev[i].N < 0) return FD2_E_INVAL;
+    if (ev[i].geom >= (uint32_t)c->n_geoms) return FD2_E_INVAL;
+  }
+  return FD2_OK;
+}
+
+// queue build + transport on events already in device memory
+int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
+            fd2_photon* d_records) {
+  if (n <= 0) return FD2_OK;
+  int rc = grow(c, (size_t)n);
+  if (rc) return rc;
+  QueueArgs q{};
+  q.events = d_ev;
+  q.chunk_off = c->d_chunk_off;
+  q.photon_off = c->d_photon_off;
+  q.block_chunks = c->d_block_chunks;
+  q.block_photons = c->d_block_photons;
+  q.cursor = c->d_cursor;
+  q.error_flag = c->d_error;
+  q.n_events = n;
+  q.n_geoms = c->n_geoms;
+  q.chunk_log2 = ilog2(c->tune.chunk);
+  q.n_blocks = (int)((n + kScanTile - 1) / kScanTile);
+
+  TransportArgs a{};
+  a.events = d_ev;
+  a.chunk_off = c->d_chunk_off;
+  a.photon_off = c->d_photon_off;
+  a.geoms = c->d_geoms;
+  a.counters = c->d_counters;
+  a.cursor = c->d_cursor;
+  a.records = d_records;
+  a.n_events = n;
+  a.seed = c->seed;
+  a.event_base = (uint32_t)c->next_event;
+  a.n_geoms = c->n_geoms;
+  a.chunk_log2 = q.chunk_log2;
+
+  const bool multi = c->n_geoms > 1;
+  const int per_cu = c->tune.waves_per_cu / (kTransportThreads / 64);
+  const int occ = transport_max_blocks_per_cu(multi, record);
+  const int blocks = c->n_cu * (per_cu < 1 ? 1 : (per_cu > occ ? occ : per_cu));
+
+  FD2_HIP(hipEventRecord(c->ev_start, c->stream));
+  FD2_HIP(launch_queue_build(q, c->stream));
+  FD2_HIP(launch_transport(a, multi, record, blocks, c->stream));
+  FD2_HIP(hipEventRecord(c->ev_stop, c->stream));
+  c->timed = true;
+  c->next_event += n;
+  return FD2_OK;
+}
+
+int check_error_flag(fd2_ctx* c) {
+  uint32_t flag = 0;
+  FD2_HIP(hipMemcpyAsync(&flag, c->d_error, sizeof flag, hipMemcpyDeviceToHost, c->stream));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  if (flag) {
+    FD2_HIP(launch_fill_u32(c->d_error, 1, c->stream));
+    FD2_HIP(hipStreamSynchronize(c->stream));
+    return FD2_E_INVAL;
+  }
+  return FD2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fd2_strerror(int status) {
+  switch (status) {
+    case FD2_OK: return "ok";
+    case FD2_E_INVAL: return "invalid argument";
+    case FD2_E_NOMEM: return "out of memory";
+    case FD2_E_DEVICE: return "HIP runtime error";
+    case FD2_E_TOO_MANY: return "event has more than 900000 optical photons";
+    case FD2_E_NO_DEVICE: return "no usable GPU";
+    case FD2_E_STATE: return "invalid call order";
+    case FD2_E_IO: return "file error";
+    default: return "unknown error";
+  }
+}
+
+int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out) {
+  if (!params || !out || n_geoms < 1 || n_geoms > FD2_MAX_GEOMS) return FD2_E_INVAL;
+  *out = nullptr;
+  for (int i = 0; i < n_geoms; ++i)
+    if (check_params(params[i])) return FD2_E_INVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FD2_E_NO_DEVICE;
+  if (device < 0 || device >= ndev) return FD2_E_INVAL;
+  fd2_ctx* c = new (std::nothrow) fd2_ctx();
+  if (!c) return FD2_E_NOMEM;
+  c->device = device;
+  c->n_geoms = n_geoms;
+  c->tune.chunk = 64;
+  c->tune.waves_per_cu = 16;
+  c->tune.lds_tile = 0;
+  int rc = FD2_OK;
+  do {
+    if (hipSetDevice(device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
+    c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = FD2_E_DEVICE; break; }
+    if (hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
+      rc = FD2_E_DEVICE;
+      break;
+    }
+    for (int i = 0; i < n_geoms; ++i) {
+      c->params[i] = params[i];
+      derive(params[i], c->hgeom[i]);
+      const size_t npix = (size_t)c->hgeom[i].xdim * c->hgeom[i].ydim;
+      const size_t nh = params[i].mynumhist > 0 ? (size_t)params[i].mynumhist : 1;
+      if (hipMalloc(&c->d_prf[i], npix * sizeof(unsigned long long)) != hipSuccess ||
+          hipMalloc(&c->d_hist[i], nh * sizeof(uint32_t)) != hipSuccess) {
+        rc = FD2_E_NOMEM;
+        break;
+      }
+      c->hgeom[i].prf = c->d_prf[i];
+      c->hgeom[i].hist = c->d_hist[i];
+    }
+    if (rc) break;
+    if (hipMalloc(&c->d_geoms, sizeof(GeomDev) * n_geoms) != hipSuccess ||
+        hipMalloc(&c->d_counters, sizeof(unsigned long long) * kCounters * n_geoms) != hipSuccess ||
+        hipMalloc(&c->d_cursor, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&c->d_error, sizeof(uint32_t)) != hipSuccess) {
+      rc = FD2_E_NOMEM;
+      break;
+    }
+    if (hipMemcpyAsync(c->d_geoms, c->hgeom, sizeof(GeomDev) * n_geoms,
+                       hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+      rc = FD2_E_DEVICE;
+      break;
+    }
+    if (launch_fill_u32(c->d_error, 1, c->stream) != hipSuccess) { rc = FD2_E_DEVICE; break; }
+    if (launch_fill_u32(c->d_cursor, 1, c->stream) != hipSuccess) { rc = FD2_E_DEVICE; break; }
+    rc = fd2_reset(c);
+  } while (0);
+  if (rc) {
+    fd2_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return FD2_OK;
+}
+
+void fd2_destroy(fd2_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < FD2_MAX_GEOMS; ++i) {
+    if (c->d_prf[i]) (void)hipFree(c->d_prf[i]);
+    if (c->d_hist[i]) (void)hipFree(c->d_hist[i]);
+  }
+  void* bufs[] = {c->d_geoms, c->d_counters, c->d_events, c->d_chunk_off,
+                  c->d_photon_off, c->d_block_chunks, c->d_block_photons,
+                  c->d_cursor, c->d_error};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+  if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int fd2_set_seed(fd2_ctx* c, uint32_t seed) {
+  if (!c) return FD2_E_INVAL;
+  c->seed = seed;
+  return FD2_OK;
+}
+
+int fd2_set_tuning(fd2_ctx* c, const fd2_tuning* t) {
+  if (!c || !t) return FD2_E_INVAL;
+  fd2_tuning n = c->tune;
+  if (t->chunk) {
+    if (t->chunk < 1 || t->chunk > (1 << 16) || (t->chunk & (t->chunk - 1))) return FD2_E_INVAL;
+    n.chunk = t->chunk;
+  }
+  if (t->waves_per_cu) {
+    if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return FD2_E_INVAL;
+    n.waves_per_cu = t->waves_per_cu;
+  }
+  if (t->lds_tile < 0) return FD2_E_INVAL;
+  n.lds_tile = t->lds_tile;
+  c->tune = n;
+  return FD2_OK;
+}
+
+int fd2_get_tuning(const fd2_ctx* c, fd2_tuning* t) {
+  if (!c || !t) return FD2_E_INVAL;
+  *t = c->tune;
+  return FD2_OK;
+}
+
+int fd2_reset(fd2_ctx* c) {
+  if (!c) return FD2_E_INVAL;
+  FD2_HIP(hipSetDevice(c->device));
+  for (int i = 0; i < c->n_geoms; ++i) {
+    const size_t npix = (size_t)c->hgeom[i].xdim * c->hgeom[i].ydim;
+    const size_t nh = c->params[i].mynumhist > 0 ? (size_t)c->params[i].mynumhist : 1;
+    FD2_HIP(launch_fill_u64(c->d_prf[i], npix, c->stream));
+    FD2_HIP(launch_fill_u32(c->d_hist[i], nh, c->stream));
+  }
+  FD2_HIP(launch_fill_u64(c->d_counters, (size_t)kCounters * c->n_geoms, c->stream));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  c->next_event = 0;
+  return FD2_OK;
+}
+
+int fd2_submit(fd2_ctx* c, const fd2_event* events, int64_t n) {
+  if (!c || n < 0 || (n > 0 && !events)) return FD2_E_INVAL;
+  if (n == 0) return FD2_OK;
+  int rc = validate_host(c, events, n);
+  if (rc) return rc;
+  FD2_HIP(hipSetDevice(c->device));
+  // the previous submit may still read the staging buffer
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  rc = grow_stage(c, (size_t)n);
+  if (rc) return rc;
+  rc = grow(c, (size_t)n);
+  if (rc) return rc;
+  std::memcpy(c->h_stage, events, (size_t)n * sizeof(fd2_event));
+  FD2_HIP(hipMemcpyAsync(c->d_events, c->h_stage, (size_t)n * sizeof(fd2_event),
+                         hipMemcpyHostToDevice, c->stream));
+  return enqueue(c, c->d_events, n, false, nullptr);
+}
+
+int fd2_submit_device(fd2_ctx* c, const fd2_event* d_events, int64_t n) {
+  if (!c || n < 0 || (n > 0 && !d_events)) return FD2_E_INVAL;
+  FD2_HIP(hipSetDevice(c->device));
+  return enqueue(c, d_events, n, false, nullptr);
+}
+
+int fd2_sync(fd2_ctx* c) {
+  if (!c) return FD2_E_INVAL;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  return check_error_flag(c);
+}
+
+int fd2_submit_record(fd2_ctx* c, const fd2_event* events, int64_t n,
+                      fd2_photon* records) {
+  if (!c || n < 0 || (n > 0 && (!events || !records))) return FD2_E_INVAL;
+  if (n == 0) return FD2_OK;
+  int rc = validate_host(c, events, n);
+  if (rc) return rc;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  int64_t total = 0;
+  for (int64_t i = 0; i < n; ++i) total += events[i].N;
+  rc = grow(c, (size_t)n);
+  if (rc) return rc;
+  FD2_HIP(hipMemcpyAsync(c->d_events, events, (size_t)n * sizeof(fd2_event),
+                         hipMemcpyHostToDevice, c->stream));
+  fd2_photon* d_rec = nullptr;
+  if (total > 0) FD2_HIP(hipMalloc(&d_rec, (size_t)total * sizeof(fd2_photon)));
+  rc = enqueue(c, c->d_events, n, true, d_rec);
+  if (rc == FD2_OK && total > 0)
+    rc = hipMemcpyAsync(records, d_rec, (size_t)total * sizeof(fd2_photon),
+                        hipMemcpyDeviceToHost, c->stream) == hipSuccess ? FD2_OK : FD2_E_DEVICE;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) rc = FD2_E_DEVICE;
+  if (d_rec) (void)hipFree(d_rec);
+  if (rc) return rc;
+  return check_error_flag(c);
+}
+
+int fd2_trace_ranecu(fd2_ctx* c, const fd2_event* events, int64_t n,
+                     int64_t event_base, int32_t seed_input, fd2_photon* records) {
+  if (!c || n < 0 || (n > 0 && !events)) return FD2_E_INVAL;
+  if (n == 0) return FD2_OK;
+  int rc = validate_host(c, events, n);
+  if (rc) return rc;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  rc = grow(c, (size_t)n);
+  if (rc) return rc;
+  unsigned long long* h_off = (unsigned long long*)std::malloc((size_t)(n + 1) * sizeof(unsigned long long));
+  if (!h_off) return FD2_E_NOMEM;
+  unsigned long long acc = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    h_off[i] = acc;
+    acc += (unsigned long long)events[i].N;
+  }
+  h_off[n] = acc;
+  fd2_photon* d_rec = nullptr;
+  rc = FD2_OK;
+  if (hipMemcpyAsync(c->d_events, events, (size_t)n * sizeof(fd2_event), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->d_photon_off, h_off, (size_t)(n + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = FD2_E_DEVICE;
+  if (!rc && records && acc > 0 && hipMalloc(&d_rec, acc * sizeof(fd2_photon)) != hipSuccess) rc = FD2_E_NOMEM;
+  if (!rc && launch_trace_ranecu(c->d_events, n, event_base, seed_input, c->d_geoms, c->n_geoms,
+                                 c->d_counters, c->d_photon_off, d_rec, c->stream) != hipSuccess)
+    rc = FD2_E_DEVICE;
+  if (!rc && d_rec &&
+      hipMemcpyAsync(records, d_rec, acc * sizeof(fd2_photon), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = FD2_E_DEVICE;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) rc = FD2_E_DEVICE;
+  if (d_rec) (void)hipFree(d_rec);
+  std::free(h_off);
+  return rc;
+}
+
+int fd2_debug_steps(fd2_ctx* c, const fd2_event* events, int64_t n, int32_t max_photons,
+                    fd2_step* steps, int32_t max_steps, int32_t* n_steps) {
+  if (!c || !events || n < 1 || !steps || max_steps < 1 || !n_steps || max_photons < 0)
+    return FD2_E_INVAL;
+  int rc = validate_host(c, events, 1);
+  if (rc) return rc;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  rc = grow(c, 1);
+  if (rc) return rc;
+  float* d_steps = nullptr;
+  int32_t* d_count = nullptr;
+  static_assert(sizeof(fd2_step) == 14 * sizeof(float), "fd2_step layout");
+  if (hipMalloc(&d_steps, (size_t)max_steps * sizeof(fd2_step)) != hipSuccess) return FD2_E_NOMEM;
+  if (hipMalloc(&d_count, sizeof(int32_t)) != hipSuccess) {
+    (void)hipFree(d_steps);
+    return FD2_E_NOMEM;
+  }
+  rc = FD2_OK;
+  int32_t cnt = 0;
+  if (hipMemcpyAsync(c->d_events, events, sizeof(fd2_event), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      launch_fill_u32((uint32_t*)d_count, 1, c->stream) != hipSuccess ||
+      launch_debug_steps(c->d_events, (uint32_t)c->next_event, c->seed, c->d_geoms, max_photons,
+                         d_steps, d_count, max_steps, c->stream) != hipSuccess ||
+      hipMemcpyAsync(&cnt, d_count, sizeof cnt, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = FD2_E_DEVICE;
+  if (!rc) {
+    if (cnt > max_steps) cnt = max_steps;
+    if (cnt > 0 && hipMemcpy(steps, d_steps, (size_t)cnt * sizeof(fd2_step), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = FD2_E_DEVICE;
+    *n_steps = cnt;
+  }
+  (void)hipFree(d_steps);
+  (void)hipFree(d_count);
+  return rc;
+}
+
+int fd2_prf_dims(const fd2_ctx* c, int geom, int32_t* xdim, int32_t* ydim) {
+  if (!c || geom < 0 || geom >= c->n_geoms || !xdim || !ydim) return FD2_E_INVAL;
+  *xdim = c->hgeom[geom].xdim;
+  *ydim = c->hgeom[geom].ydim;
+  return FD2_OK;
+}
+
+int fd2_get_prf(fd2_ctx* c, int geom, uint64_t* prf) {
+  if (!c || geom < 0 || geom >= c->n_geoms || !prf) return FD2_E_INVAL;
+  int rc = fd2_sync(c);
+  if (rc) return rc;
+  const size_t npix = (size_t)c->hgeom[geom].xdim * c->hgeom[geom].ydim;
+  FD2_HIP(hipMemcpy(prf, c->d_prf[geom], npix * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return FD2_OK;
+}
+
+int fd2_get_hist_counts(fd2_ctx* c, int geom, uint32_t* hist) {
+  if (!c || geom < 0 || geom >= c->n_geoms || !hist) return FD2_E_INVAL;
+  int rc = fd2_sync(c);
+  if (rc) return rc;
+  const int32_t nh = c->params[geom].mynumhist;
+  if (nh > 0) FD2_HIP(hipMemcpy(hist, c->d_hist[geom], (size_t)nh * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return FD2_OK;
+}
+
+int fd2_get_counters(fd2_ctx* c, int geom, uint64_t* counters) {
+  if (!c || geom < 0 || geom >= c->n_geoms || !counters) return FD2_E_INVAL;
+  int rc = fd2_sync(c);
+  if (rc) return rc;
+  FD2_HIP(hipMemcpy(counters, c->d_counters + (size_t)geom * kCounters,
+                    kCounters * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return FD2_OK;
+}
+
+// PHS rule of main/hybridMANTIS_cuda_ver1_0.cu:376-399 (same in
+// main/hybridMANTIS_c_ver1_0.c:343-366): binsize = (max-min)/nbins in integer
+// arithmetic; histories without detections are skipped; <= min -> bin 0,
+// >= max -> last bin, else count/binsize - min/binsize.  A bin index past
+// the last bin (bin size not dividing max-min) is clamped to the last bin.
+int fd2_phs_from_counts(const uint32_t* hist, int32_t num_hist, int32_t min_ph,
+                        int32_t max_ph, int32_t num_bins, int32_t* phs) {
+  if (!hist || !phs || num_hist < 0 || num_bins <= 0) return FD2_E_INVAL;
+  const int binsize = (max_ph - min_ph) / num_bins;
+  if (binsize <= 0) return FD2_E_INVAL;
+  const int bincorr = min_ph / binsize;
+  for (int32_t h = 0; h < num_hist; ++h) {
+    const int64_t v = hist[h];
+    if (v <= 0) continue;
+    if (v <= min_ph) {
+      phs[0]++;
+    } else if (v >= max_ph) {
+      phs[num_bins - 1]++;
+    } else {
+      int64_t b = v / binsize - bincorr;
+      if (b > num_bins - 1) b = num_bins - 1;
+      if (b < 0) b = 0;
+      phs[b]++;
+    }
+  }
+  return FD2_OK;
+}
+
+int fd2_get_phs(fd2_ctx* c, int geom, int32_t* phs) {
+  if (!c || geom < 0 || geom >= c->n_geoms || !phs) return FD2_E_INVAL;
+  const fd2_params& p = c->params[geom];
+  if (p.mynumbins <= 0) return FD2_E_INVAL;
+  const int32_t nh = p.mynumhist;
+  uint32_t* h = (uint32_t*)std::malloc((size_t)(nh > 0 ? nh : 1) * sizeof(uint32_t));
+  if (!h) return FD2_E_NOMEM;
+  int rc = fd2_get_hist_counts(c, geom, h);
+  if (!rc) {
+    std::memset(phs, 0, (size_t)p.mynumbins * sizeof(int32_t));
+    rc = fd2_phs_from_counts(h, nh, p.minphotons, p.maxphotons, p.mynumbins, phs);
+  }
+  std::free(h);
+  return rc;
+}
+
+int fd2_device_alloc(fd2_ctx* c, size_t bytes, void** d_ptr) {
+  if (!c || !d_ptr || bytes == 0) return FD2_E_INVAL;
+  FD2_HIP(hipSetDevice(c->device));
+  if (hipMalloc(d_ptr, bytes) != hipSuccess) return FD2_E_NOMEM;
+  return FD2_OK;
+}
+
+int fd2_device_free(fd2_ctx* c, void* d_ptr) {
+  if (!c) return FD2_E_INVAL;
+  if (!d_ptr) return FD2_OK;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  FD2_HIP(hipFree(d_ptr));
+  return FD2_OK;
+}
+
+int fd2_copy_to_device(fd2_ctx* c, void* d_dst, const void* h_src, size_t bytes) {
+  if (!c || !d_dst || (!h_src && bytes)) return FD2_E_INVAL;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  FD2_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  return FD2_OK;
+}
+
+int fd2_last_kernel_ms(fd2_ctx* c, float* ms) {
+  if (!c || !ms) return FD2_E_INVAL;
+  if (!c->timed) return FD2_E_STATE;
+  FD2_HIP(hipEventSynchronize(c->ev_stop));
+  FD2_HIP(hipEventElapsedTime(ms, c->ev_start, c->ev_stop));
+  return FD2_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// fd2_internal.h -- declarations shared by the host library and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fd2.h"
+#include "fd2_device.h"
+
+namespace fd2 {
+
+enum { kCounters = FD2_NCOUNT };
+
+struct TransportArgs {
+  const fd2_event* events;             // device
+  const uint32_t* chunk_off;           // n_events + 1 (exclusive scan)
+  const unsigned long long* photon_off;  // n_events + 1 (exclusive scan)
+  const GeomDev* geoms;                // n_geoms
+  unsigned long long* counters;        // n_geoms * FD2_NCOUNT
+  uint32_t* cursor;                    // work-queue cursor (zeroed by the scan)
+  fd2_photon* records;                 // per-photon records (recording only)
+  int64_t n_events;
+  uint32_t seed;
+  uint32_t event_base;                 // global number of events[0]
+  int32_t n_geoms;
+  int32_t chunk_log2;
+};
+
+struct QueueArgs {
+  const fd2_event* events;
+  uint32_t* chunk_off;
+  unsigned long long* photon_off;
+  uint32_t* block_chunks;      // per scan block
+  unsigned long long* block_photons;
+  uint32_t* cursor;
+  uint32_t* error_flag;        // set when an event is invalid
+  int64_t n_events;
+  int32_t n_geoms;
+  int32_t chunk_log2;
+  int32_t n_blocks;
+};
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kTransportThreads = 256;
+
+// launch wrappers (fd2_kernels.hip)
+hipError_t launch_queue_build(const QueueArgs& q, hipStream_t s);
+hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
+                            int n_blocks, hipStream_t s);
+hipError_t launch_trace_ranecu(const fd2_event* events, int64_t n,
+                               int64_t event_base, int32_t seed_input,
+                               const GeomDev* geoms, int32_t n_geoms,
+                               unsigned long long* counters,
+                               const unsigned long long* photon_off,
+                               fd2_photon* records, hipStream_t s);
+hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
+                              uint32_t seed, const GeomDev* geoms,
+                              int32_t n_photons, float* steps,
+                              int32_t* n_steps, int32_t max_steps,
+                              hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, size_t n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
+int transport_max_blocks_per_cu(bool multi, bool record);
+
+}  // namespace fd2

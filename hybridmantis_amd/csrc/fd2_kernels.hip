@@ -1,0 +1,460 @@
+// fd2_kernels.hip -- HIP kernels of the MI355X fastDETECT2 transport.
+//
+//   k_queue_*     device work queue: exclusive scans of per-event chunk and
+//                 photon counts (replaces the reference's host load
+//                 balancer, main/hybridMANTIS_tallyEnergyDepositionEvents.f:
+//                 195-250, and its one-thread-per-event launch,
+//                 main/hybridMANTIS_cuda_ver1_0.cu:277)
+//   k_transport   persistent kernel: each wave takes chunks of photons from
+//                 the queue with one atomic per chunk and hands photons to
+//                 its idle lanes, so events of very uneven size keep every
+//                 lane busy
+//   k_trace_ranecu one lane per event, photons in sequence, the reference's
+//                 per-event RANECU stream (trajectory test)
+//   k_debug_steps per-step FP32 trace of a few photons (FP64 re-trace check)
+#include "fd2_internal.h"
+
+namespace fd2 {
+
+// ------------------------------------------------------------ helpers
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// exclusive scan over the block (kScanThreads threads); returns the
+// exclusive prefix of `v` and the block total in `total`
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* s_wave, T& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int kWaves = kScanThreads / 64;
+  T inc = wave_incl_scan(v);
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  if (wid == 0) {
+    T w = lane < kWaves ? s_wave[lane] : T(0);
+    T wi = wave_incl_scan(w);
+    if (lane < kWaves) s_wave[lane] = wi - w;
+    if (lane == kWaves - 1) s_wave[kWaves] = wi;
+  }
+  __syncthreads();
+  total = s_wave[kWaves];
+  T r = s_wave[wid] + inc - v;
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ void event_counts(const QueueArgs& q, int64_t e,
+                                             uint32_t& chunks, uint32_t& photons) {
+  chunks = 0;
+  photons = 0;
+  if (e >= q.n_events) return;
+  const fd2_event ev = q.events[e];
+  if (ev.N < 0 || ev.N > FD2_MAX_PHOTONS_PER_EVENT || ev.geom >= (uint32_t)q.n_geoms) {
+    if (ev.N != 0) atomicOr(q.error_flag, 1u);
+    return;
+  }
+  photons = (uint32_t)ev.N;
+  chunks = (photons + (1u << q.chunk_log2) - 1u) >> q.chunk_log2;
+}
+
+// pass 1: per-block totals
+__global__ void __launch_bounds__(kScanThreads) k_queue_totals(QueueArgs q) {
+  __shared__ uint32_t s_c[kScanThreads / 64 + 1];
+  __shared__ unsigned long long s_p[kScanThreads / 64 + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  uint32_t c = 0;
+  unsigned long long ph = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    uint32_t cc, pp;
+    event_counts(q, base + i, cc, pp);
+    c += cc;
+    ph += pp;
+  }
+  uint32_t tc;
+  unsigned long long tp;
+  block_excl_scan<uint32_t>(c, s_c, tc);
+  block_excl_scan<unsigned long long>(ph, s_p, tp);
+  if (threadIdx.x == 0) {
+    q.block_chunks[blockIdx.x] = tc;
+    q.block_photons[blockIdx.x] = tp;
+  }
+}
+
+// pass 2: scan of the block totals (one block), grand totals, cursor reset
+__global__ void __launch_bounds__(kScanThreads) k_queue_blocks(QueueArgs q) {
+  __shared__ uint32_t s_c[kScanThreads / 64 + 1];
+  __shared__ unsigned long long s_p[kScanThreads / 64 + 1];
+  uint32_t carry_c = 0;
+  unsigned long long carry_p = 0;
+  for (int b0 = 0; b0 < q.n_blocks; b0 += kScanThreads) {
+    const int b = b0 + threadIdx.x;
+    const uint32_t c = b < q.n_blocks ? q.block_chunks[b] : 0u;
+    const unsigned long long p = b < q.n_blocks ? q.block_photons[b] : 0ull;
+    uint32_t tc;
+    unsigned long long tp;
+    const uint32_t ec = block_excl_scan<uint32_t>(c, s_c, tc);
+    const unsigned long long ep = block_excl_scan<unsigned long long>(p, s_p, tp);
+    if (b < q.n_blocks) {
+      q.block_chunks[b] = carry_c + ec;
+      q.block_photons[b] = carry_p + ep;
+    }
+    carry_c += tc;
+    carry_p += tp;
+  }
+  if (threadIdx.x == 0) {
+    q.chunk_off[q.n_events] = carry_c;
+    q.photon_off[q.n_events] = carry_p;
+    *q.cursor = 0u;
+  }
+}
+
+// pass 3: per-event exclusive offsets
+__global__ void __launch_bounds__(kScanThreads) k_queue_offsets(QueueArgs q) {
+  __shared__ uint32_t s_c[kScanThreads / 64 + 1];
+  __shared__ unsigned long long s_p[kScanThreads / 64 + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  uint32_t cc[kScanItems], pp[kScanItems];
+  uint32_t c = 0;
+  unsigned long long ph = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    event_counts(q, base + i, cc[i], pp[i]);
+    c += cc[i];
+    ph += pp[i];
+  }
+  uint32_t tc;
+  unsigned long long tp;
+  uint32_t oc = block_excl_scan<uint32_t>(c, s_c, tc) + q.block_chunks[blockIdx.x];
+  unsigned long long op =
+      block_excl_scan<unsigned long long>(ph, s_p, tp) + q.block_photons[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const int64_t e = base + i;
+    if (e < q.n_events) {
+      q.chunk_off[e] = oc;
+      q.photon_off[e] = op;
+    }
+    oc += cc[i];
+    op += pp[i];
+  }
+}
+
+hipError_t launch_queue_build(const QueueArgs& q, hipStream_t s) {
+  if (q.n_blocks <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_queue_totals, dim3(q.n_blocks), dim3(kScanThreads), 0, s, q);
+  hipLaunchKernelGGL(k_queue_blocks, dim3(1), dim3(kScanThreads), 0, s, q);
+  hipLaunchKernelGGL(k_queue_offsets, dim3(q.n_blocks), dim3(kScanThreads), 0, s, q);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ tallies
+
+__device__ __forceinline__ int fate_counter(int fate) {
+  switch (fate) {
+    case kDetect:
+    case kDetectOut: return FD2_C_DETECT;
+    case kAbsTop: return FD2_C_ABS_TOP;
+    case kAbsBulk: return FD2_C_ABS_BULK;
+    case kLost: return FD2_C_LOST;
+    case kOutOfCol: return FD2_C_OUTOFCOL;
+    case kTheta1: return FD2_C_THETA1;
+    default: return FD2_C_REGEN;  // unreachable for an ended photon
+  }
+}
+
+__device__ __forceinline__ void start_photon(Photon& p, const fd2_event& ev) {
+  p.ex = ev.x;
+  p.ey = ev.y;
+  p.ez = ev.z;
+  p.px = ev.x;
+  p.py = ev.y;
+  p.pz = ev.z;
+  p.hist = ev.hist;
+  p.first = true;
+  p.loops = 0;
+  p.fate = kNone;
+  p.pix = -1;
+  p.fx = 0.0f;
+  p.fy = 0.0f;
+  p.nx = p.ny = p.nz = 0.0f;
+  p.xc = ev.x;
+  p.yc = ev.y;
+}
+
+// Counter-mode initial direction of photon k: photon 0 takes the driver's
+// first direction (|2u-1| > 0.01, main/hybridMANTIS_c_ver1_0.c:257-277),
+// the others the direction algo() draws at the previous photon's end
+// (main/kernel_cuda_c_ver1_0.cu:578-602) followed by the RoughSurface draws
+// of the (never read) initial normal.
+__device__ __forceinline__ void counter_initial_dir(CounterRng& rng, Photon& p,
+                                                    uint32_t k, const GeomDev& g) {
+  if (k == 0) {
+    emit_first_dir(rng, p, g);
+  } else {
+    emit_dir(rng, p, g);
+    if (g.beta > 0.0f) {
+      (void)rng.u();
+      (void)rng.u();
+    }
+  }
+}
+
+// ------------------------------------------------------- transport
+
+template <bool MULTI, bool REC>
+__global__ void __launch_bounds__(kTransportThreads) k_transport(TransportArgs a) {
+  __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
+  for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x) s_cnt[i] = 0;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const uint32_t chunk = 1u << a.chunk_log2;
+
+  // wave-uniform work pool: photons [pool_next, pool_end) of event pool_ev
+  int64_t pool_ev = 0;
+  int32_t pool_next = 0, pool_end = 0;
+  bool pool_done = false;
+
+  Photon p;
+  CounterRng rng;
+  bool alive = false;
+  uint32_t gi = 0;
+  unsigned long long rec_idx = 0;
+  StepSink nosink{nullptr, nullptr, 0, 0, false};
+
+  for (;;) {
+    unsigned long long need = __ballot(!alive);
+    while (need != 0ull && !pool_done) {
+      if (pool_next >= pool_end) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(a.cursor, 1u);
+        c = (uint32_t)__builtin_amdgcn_readfirstlane((int)c);
+        if (c >= a.chunk_off[a.n_events]) {
+          pool_done = true;
+          break;
+        }
+        // largest e with chunk_off[e] <= c (skips events without photons)
+        int64_t lo = 0, hi = a.n_events;
+        while (hi - lo > 1) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (a.chunk_off[mid] <= c) lo = mid; else hi = mid;
+        }
+        pool_ev = lo;
+        const int32_t n_ph = a.events[lo].N;
+        pool_next = (int32_t)((c - a.chunk_off[lo]) << a.chunk_log2);
+        pool_end = min(n_ph, pool_next + (int32_t)chunk);
+        if (lane == 0) {
+          const uint32_t gg = MULTI ? a.events[lo].geom : 0u;
+          atomicAdd(&s_cnt[gg * kCounters + FD2_C_GENERATED], (uint32_t)(pool_end - pool_next));
+        }
+      }
+      const int avail = pool_end - pool_next;
+      const int rank = __popcll(need & lt_mask);
+      if (!alive && rank < avail) {
+        const fd2_event ev = a.events[pool_ev];
+        gi = MULTI ? ev.geom : 0u;
+        const GeomDev& g = a.geoms[gi];
+        const uint32_t k = (uint32_t)(pool_next + rank);
+        start_photon(p, ev);
+        rng.seed(a.seed, a.event_base + (uint32_t)pool_ev, k);
+        counter_initial_dir(rng, p, k, g);
+        if (REC) rec_idx = a.photon_off[pool_ev] + k;
+        alive = true;
+      }
+      pool_next += min(__popcll(need), avail);
+      need = __ballot(!alive);
+    }
+    if (__ballot(alive) == 0ull) break;
+    if (alive) {
+      const GeomDev& g = a.geoms[MULTI ? gi : 0u];
+      const int r = photon_step<CounterRng, false>(rng, p, g, &nosink);
+      if (r == kEnded) {
+        atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
+        if (REC) {
+          fd2_photon rec;
+          rec.fate = p.fate;
+          rec.pix = p.pix;
+          rec.x = p.fx;
+          rec.y = p.fy;
+          a.records[rec_idx] = rec;
+        }
+        alive = false;
+      } else if (r == kTrapped) {
+        atomicAdd(&s_cnt[gi * kCounters + FD2_C_REGEN], 1u);
+        restart_trapped(rng, p, g);
+      }
+    }
+  }
+
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x)
+    if (s_cnt[i]) atomicAdd(&a.counters[i], (unsigned long long)s_cnt[i]);
+}
+
+hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
+                            int n_blocks, hipStream_t s) {
+  dim3 grid(n_blocks), block(kTransportThreads);
+  if (multi) {
+    if (record) hipLaunchKernelGGL((k_transport<true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_transport<true, false>), grid, block, 0, s, a);
+  } else {
+    if (record) hipLaunchKernelGGL((k_transport<false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_transport<false, false>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+int transport_max_blocks_per_cu(bool multi, bool record) {
+  int nb = 0;
+  hipError_t e;
+  if (multi) {
+    e = record ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<true, true>, kTransportThreads, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<true, false>, kTransportThreads, 0);
+  } else {
+    e = record ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<false, true>, kTransportThreads, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<false, false>, kTransportThreads, 0);
+  }
+  return e == hipSuccess && nb > 0 ? nb : 1;
+}
+
+// ------------------------------------------------ RANECU trajectory mode
+
+__global__ void __launch_bounds__(64) k_trace_ranecu(
+    const fd2_event* events, int64_t n, int64_t event_base, int32_t seed_input,
+    const GeomDev* geoms, unsigned long long* counters,
+    const unsigned long long* photon_off, fd2_photon* records) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const fd2_event ev = events[i];
+  const GeomDev& g = geoms[ev.geom];
+  uint32_t cnt[kCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
+  RanecuRng rng;
+  ranecu_init(rng, (int)(event_base + i), seed_input);
+  StepSink nosink{nullptr, nullptr, 0, 0, false};
+  Photon p;
+  start_photon(p, ev);
+  if (ev.N > 0) emit_first_dir(rng, p, g);  // driver, normal stays 0
+  for (int32_t k = 0; k < ev.N; ++k) {
+    p.px = ev.x;
+    p.py = ev.y;
+    p.pz = ev.z;
+    p.first = true;
+    p.loops = 0;
+    p.fate = kNone;
+    p.pix = -1;
+    cnt[FD2_C_GENERATED]++;
+    for (;;) {
+      const int r = photon_step<RanecuRng, false>(rng, p, g, &nosink);
+      if (r == kEnded) break;
+      if (r == kTrapped) {
+        cnt[FD2_C_REGEN]++;
+        restart_trapped(rng, p, g);
+      }
+    }
+    cnt[fate_counter(p.fate)]++;
+    if (records) {
+      fd2_photon rec;
+      rec.fate = p.fate;
+      rec.pix = p.pix;
+      rec.x = p.fx;
+      rec.y = p.fy;
+      records[photon_off[i] + k] = rec;
+    }
+    // next photon's direction and normal draws (main/kernel_cuda_c_ver1_0.cu:578-602)
+    emit_dir(rng, p, g);
+    if (g.beta > 0.0f) {
+      (void)rng.u();
+      (void)rng.u();
+    }
+  }
+  unsigned long long* c = counters + (size_t)ev.geom * kCounters;
+  for (int j = 0; j < kCounters; ++j)
+    if (cnt[j]) atomicAdd(&c[j], (unsigned long long)cnt[j]);
+}
+
+hipError_t launch_trace_ranecu(const fd2_event* events, int64_t n,
+                               int64_t event_base, int32_t seed_input,
+                               const GeomDev* geoms, int32_t n_geoms,
+                               unsigned long long* counters,
+                               const unsigned long long* photon_off,
+                               fd2_photon* records, hipStream_t s) {
+  (void)n_geoms;
+  if (n <= 0) return hipSuccess;
+  const int64_t nb = (n + 63) / 64;
+  hipLaunchKernelGGL(k_trace_ranecu, dim3((unsigned)nb), dim3(64), 0, s, events, n,
+                     event_base, seed_input, geoms, counters, photon_off, records);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------- step trace (debug)
+
+__global__ void __launch_bounds__(64) k_debug_steps(
+    const fd2_event* events, uint32_t event_base, uint32_t seed,
+    const GeomDev* geoms, int32_t n_photons, float* steps, int32_t* n_steps,
+    int32_t max_steps) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const fd2_event ev = events[0];
+  if (k >= n_photons || k >= ev.N) return;
+  const GeomDev& g = geoms[ev.geom];
+  StepSink sink{steps, n_steps, max_steps, k, true};
+  Photon p;
+  CounterRng rng;
+  start_photon(p, ev);
+  rng.seed(seed, event_base, (uint32_t)k);
+  counter_initial_dir(rng, p, (uint32_t)k, g);
+  for (;;) {
+    const int r = photon_step<CounterRng, true>(rng, p, g, &sink);
+    if (r == kEnded) break;
+    if (r == kTrapped) restart_trapped(rng, p, g);
+  }
+}
+
+hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
+                              uint32_t seed, const GeomDev* geoms,
+                              int32_t n_photons, float* steps,
+                              int32_t* n_steps, int32_t max_steps,
+                              hipStream_t s) {
+  if (n_photons <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_debug_steps, dim3((n_photons + 63) / 64), dim3(64), 0, s,
+                     events, event_base, seed, geoms, n_photons, steps, n_steps,
+                     max_steps);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- fills
+
+__global__ void k_fill_u64(unsigned long long* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+__global__ void k_fill_u32(uint32_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t nb = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t nb = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace fd2

@@ -1,0 +1,286 @@
+"""Python host interface over libfd2.so (ctypes).
+
+Mirrors the reference's optical-transport operator: a parameter set with the
+names of hybridMANTIS_input.in, a buffer of (x, y, z, E, history, N) events
+handed over in one call, and the reference's tallies back (PRF image,
+photons detected per history / PHS, and the /optstats/ counters).  Every
+call goes to the HIP library; if libfd2.so has not been built, importing
+this module's functions raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .events import EVENT_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfd2.so")
+
+PHOTON_DTYPE = np.dtype([("fate", "<i4"), ("pix", "<i4"), ("x", "<f4"),
+                         ("y", "<f4")])
+STEP_DTYPE = np.dtype([("pos_in", "<f4", 3), ("dcos", "<f4", 3), ("xc", "<f4"),
+                       ("yc", "<f4"), ("pos_out", "<f4", 3), ("code", "<i4"),
+                       ("photon", "<i4"), ("reserved", "<i4")])
+COUNTER_NAMES = ["generated", "detected", "abs_top", "abs_bulk", "lost",
+                 "outofcol", "theta1", "regenerated"]
+
+FD2_OK = 0
+
+
+class Fd2Error(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """fd2_params == the reference's COMMON /inputargs/ layout."""
+    _fields_ = [(n, C.c_double) for n in (
+        "detx", "dety", "detheight", "detradius", "detnC", "detnIC", "dettop",
+        "detbulk", "detbeta", "detdmin", "detdmax", "detlboundx", "detlboundy",
+        "detuboundx", "detuboundy", "detyield", "detsensorRefl")] + [
+        (n, C.c_int32) for n in ("detpixel", "rungpu", "machinenum",
+                                 "mynumhist", "minphotons", "maxphotons",
+                                 "mynumbins")]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("chunk", C.c_int32), ("waves_per_cu", C.c_int32),
+                ("lds_tile", C.c_int32), ("reserved", C.c_int32)]
+
+
+# dict keys (events.BASE_PARAMS) -> fd2_params members
+_PARAM_MAP = dict(
+    xdet="detx", ydet="dety", H="detheight", R="detradius", n_C="detnC",
+    n_IC="detnIC", top_absfrac="dettop", bulk="detbulk", beta="detbeta",
+    d_min="detdmin", d_max="detdmax", lbx="detlboundx", lby="detlboundy",
+    ubx="detuboundx", uby="detuboundy", yield_per_ev="detyield",
+    sensor_refl="detsensorRefl", pixel="detpixel", rungpu="rungpu",
+    machine="machinenum", num_hist="mynumhist", min_ph="minphotons",
+    max_ph="maxphotons", num_bins="mynumbins")
+
+
+def to_params(p: dict) -> Params:
+    out = Params()
+    for k, v in _PARAM_MAP.items():
+        if k in p:
+            setattr(out, v, p[k])
+    return out
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libfd2.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Fd2Error(f"{LIB_PATH} is missing: build it with "
+                       "`python -m hybridmantis_amd.build` (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "fd2_create": ([C.POINTER(Params), C.c_int, C.c_int, C.POINTER(P)], C.c_int),
+        "fd2_destroy": ([P], None),
+        "fd2_set_seed": ([P, C.c_uint32], C.c_int),
+        "fd2_set_tuning": ([P, C.POINTER(Tuning)], C.c_int),
+        "fd2_get_tuning": ([P, C.POINTER(Tuning)], C.c_int),
+        "fd2_submit": ([P, P, C.c_int64], C.c_int),
+        "fd2_submit_device": ([P, P, C.c_int64], C.c_int),
+        "fd2_sync": ([P], C.c_int),
+        "fd2_submit_record": ([P, P, C.c_int64, P], C.c_int),
+        "fd2_trace_ranecu": ([P, P, C.c_int64, C.c_int64, C.c_int32, P], C.c_int),
+        "fd2_debug_steps": ([P, P, C.c_int64, C.c_int32, P, C.c_int32,
+                             C.POINTER(C.c_int32)], C.c_int),
+        "fd2_prf_dims": ([P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+        "fd2_get_prf": ([P, C.c_int, P], C.c_int),
+        "fd2_get_hist_counts": ([P, C.c_int, P], C.c_int),
+        "fd2_get_phs": ([P, C.c_int, P], C.c_int),
+        "fd2_get_counters": ([P, C.c_int, P], C.c_int),
+        "fd2_reset": ([P], C.c_int),
+        "fd2_last_kernel_ms": ([P, C.POINTER(C.c_float)], C.c_int),
+        "fd2_phs_from_counts": ([P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P], C.c_int),
+        "fd2_device_alloc": ([P, C.c_size_t, C.POINTER(P)], C.c_int),
+        "fd2_device_free": ([P, P], C.c_int),
+        "fd2_copy_to_device": ([P, P, P, C.c_size_t], C.c_int),
+        "fd2_strerror": ([C.c_int], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != FD2_OK:
+        raise Fd2Error(f"{what}: {lib().fd2_strerror(rc).decode()} ({rc})")
+
+
+def as_events(ev) -> np.ndarray:
+    a = np.ascontiguousarray(ev)
+    if a.dtype != EVENT_DTYPE:
+        out = np.zeros(len(a), EVENT_DTYPE)
+        for k in a.dtype.names:
+            if k in EVENT_DTYPE.names:
+                out[k] = a[k]
+        a = out
+    return a
+
+
+def phs_from_counts(hist_counts, min_ph: int, max_ph: int, num_bins: int) -> np.ndarray:
+    """The reference's PHS binning rule (hybridMANTIS_cuda_ver1_0.cu:376-399)."""
+    h = np.ascontiguousarray(hist_counts, np.uint32)
+    out = np.zeros(num_bins, np.int32)
+    _check(lib().fd2_phs_from_counts(h.ctypes.data, len(h), min_ph, max_ph,
+                                     num_bins, out.ctypes.data), "phs")
+    return out
+
+
+class DeviceBuffer:
+    """Device allocation owned through the library (no other GPU runtime)."""
+
+    def __init__(self, tr: "Transport", nbytes: int):
+        self._tr = tr
+        self.nbytes = nbytes
+        p = C.c_void_p()
+        _check(lib().fd2_device_alloc(tr._ctx, nbytes, C.byref(p)), "device_alloc")
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        _check(lib().fd2_copy_to_device(self._tr._ctx, self.ptr, a.ctypes.data,
+                                        a.nbytes), "copy_to_device")
+
+    def free(self) -> None:
+        if self.ptr:
+            _check(lib().fd2_device_free(self._tr._ctx, self.ptr), "device_free")
+            self.ptr = None
+
+
+class Transport:
+    """One GPU context: parameter sets (geometries), tallies, work queue."""
+
+    def __init__(self, params, device: int = 0, seed: int = 271828182,
+                 tuning: dict | None = None):
+        plist = params if isinstance(params, (list, tuple)) else [params]
+        self.params = [dict(p) for p in plist]
+        arr = (Params * len(plist))(*[to_params(p) for p in plist])
+        ctx = C.c_void_p()
+        _check(lib().fd2_create(arr, len(plist), device, C.byref(ctx)), "fd2_create")
+        self._ctx = ctx
+        _check(lib().fd2_set_seed(ctx, seed & 0xFFFFFFFF), "set_seed")
+        if tuning:
+            self.set_tuning(**tuning)
+
+    # -- lifecycle
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            lib().fd2_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration
+    def set_tuning(self, chunk: int = 0, waves_per_cu: int = 0, lds_tile: int = 0):
+        t = Tuning(chunk, waves_per_cu, lds_tile, 0)
+        _check(lib().fd2_set_tuning(self._ctx, C.byref(t)), "set_tuning")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().fd2_get_tuning(self._ctx, C.byref(t)), "get_tuning")
+        return dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu, lds_tile=t.lds_tile)
+
+    def set_seed(self, seed: int) -> None:
+        _check(lib().fd2_set_seed(self._ctx, seed & 0xFFFFFFFF), "set_seed")
+
+    # -- transport
+    def submit(self, events) -> None:
+        ev = as_events(events)
+        _check(lib().fd2_submit(self._ctx, ev.ctypes.data, len(ev)), "fd2_submit")
+
+    def submit_device(self, buf: DeviceBuffer, n: int) -> None:
+        _check(lib().fd2_submit_device(self._ctx, buf.ptr, n), "fd2_submit_device")
+
+    def sync(self) -> None:
+        _check(lib().fd2_sync(self._ctx), "fd2_sync")
+
+    def submit_record(self, events) -> np.ndarray:
+        ev = as_events(events)
+        rec = np.zeros(int(ev["N"].sum()), PHOTON_DTYPE)
+        _check(lib().fd2_submit_record(self._ctx, ev.ctypes.data, len(ev),
+                                       rec.ctypes.data), "fd2_submit_record")
+        return rec
+
+    def trace_ranecu(self, events, event_base: int = 0,
+                     seed_input: int = 271828182, record: bool = True):
+        ev = as_events(events)
+        rec = np.zeros(int(ev["N"].sum()), PHOTON_DTYPE) if record else None
+        _check(lib().fd2_trace_ranecu(self._ctx, ev.ctypes.data, len(ev), event_base,
+                                      seed_input,
+                                      rec.ctypes.data if rec is not None else None),
+               "fd2_trace_ranecu")
+        return rec
+
+    def debug_steps(self, events, max_photons: int, max_steps: int) -> np.ndarray:
+        ev = as_events(events)
+        out = np.zeros(max_steps, STEP_DTYPE)
+        n = C.c_int32()
+        _check(lib().fd2_debug_steps(self._ctx, ev.ctypes.data, len(ev), max_photons,
+                                     out.ctypes.data, max_steps, C.byref(n)),
+               "fd2_debug_steps")
+        return out[:n.value]
+
+    def device_buffer(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    # -- tallies
+    def prf_dims(self, geom: int = 0):
+        x, y = C.c_int32(), C.c_int32()
+        _check(lib().fd2_prf_dims(self._ctx, geom, C.byref(x), C.byref(y)), "prf_dims")
+        return x.value, y.value
+
+    def prf(self, geom: int = 0) -> np.ndarray:
+        xd, yd = self.prf_dims(geom)
+        out = np.zeros((xd, yd), np.uint64)
+        _check(lib().fd2_get_prf(self._ctx, geom, out.ctypes.data), "get_prf")
+        return out
+
+    def hist_counts(self, geom: int = 0) -> np.ndarray:
+        out = np.zeros(max(1, int(self.params[geom]["num_hist"])), np.uint32)
+        _check(lib().fd2_get_hist_counts(self._ctx, geom, out.ctypes.data), "get_hist")
+        return out[: int(self.params[geom]["num_hist"])]
+
+    def phs(self, geom: int = 0) -> np.ndarray:
+        out = np.zeros(int(self.params[geom]["num_bins"]), np.int32)
+        _check(lib().fd2_get_phs(self._ctx, geom, out.ctypes.data), "get_phs")
+        return out
+
+    def counters(self, geom: int = 0) -> dict:
+        out = np.zeros(8, np.uint64)
+        _check(lib().fd2_get_counters(self._ctx, geom, out.ctypes.data), "get_counters")
+        return dict(zip(COUNTER_NAMES, (int(v) for v in out)))
+
+    def reset(self) -> None:
+        _check(lib().fd2_reset(self._ctx), "fd2_reset")
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        _check(lib().fd2_last_kernel_ms(self._ctx, C.byref(ms)), "last_kernel_ms")
+        return float(ms.value)

@@ -1,0 +1,213 @@
+/*
+ * fd2.h -- C ABI of the MI355X fastDETECT2 optical-photon transport library
+ * (libfd2.so).  Plain pointers and sizes only.
+ *
+ * It replaces the reference's optical-transport entry points:
+ *   gpuoptical_(int *penctr, int *myfactGPU)  main/hybridMANTIS_cuda_ver1_0.cu:107
+ *   cpuoptical_(int *gpusize)                 main/hybridMANTIS_c_ver1_0.c:81
+ *   gpuoptlb_(double *gptime, int *lbbuf)     main/hybridMANTIS_cuda_ver1_0_LB.cu:108
+ *   cpuoptlb_(double *cptime, int *cpubufsize) main/hybridMANTIS_c_ver1_0_LB.c:86
+ * Those four keep their gfortran ABI (fd2_fortran.h); this header is the
+ * native API underneath them.  All functions return FD2_OK (0) or a negative
+ * FD2_E* status; none calls exit().
+ */
+#ifndef FD2_H
+#define FD2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FD2_OK 0
+#define FD2_E_INVAL (-1)        /* bad argument                           */
+#define FD2_E_NOMEM (-2)        /* host or device allocation failed      */
+#define FD2_E_DEVICE (-3)       /* HIP runtime error                      */
+#define FD2_E_TOO_MANY (-4)     /* event with N > FD2_MAX_PHOTONS_PER_EVENT */
+#define FD2_E_NO_DEVICE (-5)    /* no usable GPU                          */
+#define FD2_E_STATE (-6)        /* call not valid in the current state    */
+#define FD2_E_IO (-7)           /* file could not be read or written      */
+
+/* drv-C/drv-G's max_photon_per_EDE (main/hybridMANTIS_c_ver1_0.c:53) */
+#define FD2_MAX_PHOTONS_PER_EVENT 900000
+/* largest PRF the reference's COMMON /outputimage/ holds (501x501) is not a
+ * limit here; a PRF may have up to this many pixels per side */
+#define FD2_MAX_PRF_SIDE 4096
+#define FD2_MAX_GEOMS 16
+
+/* Detector and tally parameters: the reference's COMMON /inputargs/ block,
+ * same members, order and types (main/kernel_cuda_c_ver1_0.cu:178-205,
+ * main/hybridMANTIS_tallyEnergyDepositionEvents.f:79-102), so a copy of
+ * inputargs_ is a valid fd2_params.  Lengths in micrometres. */
+typedef struct fd2_params {
+  double detx, dety;        /* detector extent; x in (0,detx), y in (0,dety) */
+  double detheight;         /* layer thickness H; z in (-H/2, H/2)          */
+  double detradius;         /* column radius R                              */
+  double detnC, detnIC;     /* refractive index of column / gap             */
+  double dettop;            /* top-surface absorption fraction              */
+  double detbulk;           /* bulk absorption coefficient (1/um)           */
+  double detbeta;           /* wall roughness coefficient                   */
+  double detdmin, detdmax;  /* inter-column flight range                    */
+  double detlboundx, detlboundy, detuboundx, detuboundy;  /* PRF bounds      */
+  double detyield;          /* light yield (1/eV), used by the shim only    */
+  double detsensorRefl;     /* sensor reflectivity (probability)            */
+  int32_t detpixel;         /* PRF pitch (um)                               */
+  int32_t rungpu;           /* unused by the library                        */
+  int32_t machinenum;       /* output file name tag                         */
+  int32_t mynumhist;        /* number of x-ray histories (per-history tally) */
+  int32_t minphotons, maxphotons, mynumbins;  /* PHS binning                */
+} fd2_params;
+
+/* One energy-deposition event (32 B).  Positions in um, E in eV, hist is
+ * the 1-based x-ray history, N the optical photons to transport, geom the
+ * index of its fd2_params in the context (0 for single-geometry runs).
+ * Replaces struct start_info (main/kernel_cuda_c_ver1_0.cu:67-75): the
+ * reference holds doubles but converts every use to float. */
+typedef struct fd2_event {
+  float x, y, z;
+  float E;
+  int32_t hist;
+  int32_t N;
+  uint32_t geom;
+  uint32_t flags;           /* reserved, 0                                  */
+} fd2_event;
+
+/* Per-photon record written by the recording entry points. */
+typedef struct fd2_photon {
+  int32_t fate;             /* FD2_FATE_*                                   */
+  int32_t pix;              /* i*ydim + j for FD2_FATE_DETECT, else -1      */
+  float x, y;               /* detection point                              */
+} fd2_photon;
+
+enum {
+  FD2_FATE_NONE = 0, FD2_FATE_DETECT = 1, FD2_FATE_DETECT_OUT = 2,
+  FD2_FATE_ABS_TOP = 3, FD2_FATE_ABS_BULK = 4, FD2_FATE_LOST = 5,
+  FD2_FATE_OUTOFCOL = 6, FD2_FATE_THETA1 = 7
+};
+
+/* counters, in the order of the reference's /optstats/ block
+ * (main/kernel_cuda_c_ver1_0.cu:111-121) plus trapped-photon restarts */
+enum {
+  FD2_C_GENERATED = 0, FD2_C_DETECT, FD2_C_ABS_TOP, FD2_C_ABS_BULK,
+  FD2_C_LOST, FD2_C_OUTOFCOL, FD2_C_THETA1, FD2_C_REGEN, FD2_NCOUNT
+};
+
+/* Launch shape of the persistent transport kernel. 0 = library default. */
+typedef struct fd2_tuning {
+  int32_t chunk;            /* photons per work-queue item (power of 2)     */
+  int32_t waves_per_cu;     /* resident waves per CU (workgroups of 4 waves) */
+  int32_t lds_tile;         /* side of the LDS PRF tile (0 = off)           */
+  int32_t reserved;
+} fd2_tuning;
+
+typedef struct fd2_ctx fd2_ctx;
+
+/* Create a context on HIP device `device` for n_geoms parameter sets (each
+ * gets its own PRF / per-history / counter tallies, all zeroed). */
+int fd2_create(const fd2_params *params, int n_geoms, int device,
+               fd2_ctx **out);
+void fd2_destroy(fd2_ctx *ctx);
+
+/* Seed of the per-photon streams (default 271828182, the reference's
+ * seed_input, main/hybridMANTIS_c_ver1_0.c:141). */
+int fd2_set_seed(fd2_ctx *ctx, uint32_t seed);
+int fd2_set_tuning(fd2_ctx *ctx, const fd2_tuning *tuning);
+int fd2_get_tuning(const fd2_ctx *ctx, fd2_tuning *tuning);
+
+/* Transport n events.  fd2_submit copies the host array before returning
+ * (the caller may reuse it at once) and returns without waiting for the GPU;
+ * fd2_submit_device takes events already in device memory (they must stay
+ * valid until fd2_sync).  Events are numbered consecutively across submits
+ * since creation/reset; that number keys the photons' random streams, so
+ * results do not depend on how the events are split into submits. */
+int fd2_submit(fd2_ctx *ctx, const fd2_event *events, int64_t n);
+int fd2_submit_device(fd2_ctx *ctx, const fd2_event *d_events, int64_t n);
+int fd2_sync(fd2_ctx *ctx);
+
+/* Same as fd2_submit but synchronous, and writes one record per photon
+ * (Σ N records, events in order, photons in order). */
+int fd2_submit_record(fd2_ctx *ctx, const fd2_event *events, int64_t n,
+                      fd2_photon *records);
+
+/* Trajectory-test mode: one GPU lane per event, photons in sequence, one
+ * RANECU stream per event seeded as the reference C driver does
+ * (init_PRNG(event_base + i, 50000, seed_input),
+ * main/hybridMANTIS_c_ver1_0.c:237-238).  Synchronous; tallies accumulate
+ * like fd2_submit; records may be NULL. */
+int fd2_trace_ranecu(fd2_ctx *ctx, const fd2_event *events, int64_t n,
+                     int64_t event_base, int32_t seed_input,
+                     fd2_photon *records);
+
+/* Debug: the in-column steps (dist_to_surface) of the first `max_photons`
+ * photons of events[0], FP32 state before and after each step; returns the
+ * number of steps written (<= max_steps) in *n_steps.  Synchronous. */
+typedef struct fd2_step {
+  float pos_in[3], dcos[3], xc, yc, pos_out[3];
+  int32_t code;             /* 0 moved, 1 lost, 2 out of column              */
+  int32_t photon;
+  int32_t reserved;
+} fd2_step;
+int fd2_debug_steps(fd2_ctx *ctx, const fd2_event *events, int64_t n,
+                    int32_t max_photons, fd2_step *steps, int32_t max_steps,
+                    int32_t *n_steps);
+
+/* Tallies of geometry `geom` (synchronises first).  prf: xdim*ydim u64,
+ * [i][j] row-major with i the x pixel; hist: mynumhist u32 detections per
+ * history; phs: mynumbins bins with the reference's rule
+ * (main/hybridMANTIS_cuda_ver1_0.cu:376-399); counters: FD2_NCOUNT u64. */
+int fd2_prf_dims(const fd2_ctx *ctx, int geom, int32_t *xdim, int32_t *ydim);
+int fd2_get_prf(fd2_ctx *ctx, int geom, uint64_t *prf);
+int fd2_get_hist_counts(fd2_ctx *ctx, int geom, uint32_t *hist);
+int fd2_get_phs(fd2_ctx *ctx, int geom, int32_t *phs);
+int fd2_get_counters(fd2_ctx *ctx, int geom, uint64_t *counters);
+/* zero all tallies and restart event numbering */
+int fd2_reset(fd2_ctx *ctx);
+
+/* Device memory helpers for callers that keep events resident in HBM
+ * (fd2_submit_device) without a GPU runtime of their own. */
+int fd2_device_alloc(fd2_ctx *ctx, size_t bytes, void **d_ptr);
+int fd2_device_free(fd2_ctx *ctx, void *d_ptr);
+int fd2_copy_to_device(fd2_ctx *ctx, void *d_dst, const void *h_src,
+                       size_t bytes);
+
+/* Elapsed GPU time (ms) of the last transport launch sequence (queue build
+ * + persistent kernel), measured with HIP events on the library's stream. */
+int fd2_last_kernel_ms(fd2_ctx *ctx, float *ms);
+
+/* PHS from per-history counts, host side (same rule as fd2_get_phs);
+ * accumulates into phs.  FD2_E_INVAL when (max-min)/num_bins rounds to 0. */
+int fd2_phs_from_counts(const uint32_t *hist, int32_t num_hist, int32_t min_ph,
+                        int32_t max_ph, int32_t num_bins, int32_t *phs);
+
+const char *fd2_strerror(int status);
+
+/* ---- reference file formats -------------------------------------------- */
+
+/* hybridMANTIS_input.in: 24 positional tokens, first token of every
+ * non-comment, non-blank line (main/hybridMANTIS_penEasy.f:190-229).  The
+ * ten tokens used in output file names are returned verbatim in `tokens`
+ * (10 x 16 bytes: H, R, top, bulk, beta, dmax, sensorRefl, yield, pixel,
+ * machine), may be NULL. */
+int fd2_read_input(const char *path, fd2_params *out, char tokens[10][16]);
+/* myimage_<tokens>.dat / detected_<tokens>.dat names
+ * (main/hybridMANTIS_tallyEnergyDepositionEvents.f:421-431) */
+int fd2_output_names(const char tokens[10][16], char *prf_name, size_t prf_len,
+                     char *phs_name, size_t phs_len);
+/* PRF writer (tally .f:418, 448-453): pixel blocks of (1x,F12.4) values of
+ * count/n_hist, y outer and x inner, each block followed by a blank line,
+ * int((ub-lb)/pitch) pixels per side.  Fails if the file exists. */
+int fd2_write_prf(const char *path, const fd2_params *p, const uint64_t *prf,
+                  int32_t xdim, int32_t ydim, double n_hist);
+/* PHS writer (tally .f:456-458): mynumbins lines of (1x,I8). */
+int fd2_write_phs(const char *path, const int32_t *phs, int32_t num_bins);
+/* EDE text records (3(1x,1pe16.9),1x,1pe12.4,i8): x y z (cm) de (eV)
+ * history (tally .f:173-174).  Reads up to max events, converts cm to um,
+ * sets N = 0 (the caller samples the photon count); *n gets the count. */
+int fd2_read_ede(const char *path, fd2_event *events, int64_t max, int64_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
