@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X fastDETECT2 optical transport.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (default): 100,000 pencil-beam x-ray histories at 25 keV on a
+150-um CsI layer with 2.55-um columns (~2.0e5 deposition events, ~1.1e8
+optical photons, synthetic events), the plain Monte Carlo transport.  A step
+is one pass of the transport over that batch: device work-queue build plus
+the persistent transport kernel, events already resident in HBM.  The JSON
+line reports optical photons/s over K timed steps, the FP32-VALU roofline of
+the transport kernel (algorithmic FLOPs from hybridmantis_amd/flops.py with
+per-photon primitive counts measured by the oracle on a slice of the same
+workload) and a CPU baseline: the reference's own C transport branch
+(oracle/_ref, or the oracle restatement when that is not built) run on a
+bounded sample in one process per host core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "optical photons/s (plain MC); histories/s surrogate vs MC at <=2% PRF error"
+FP32_PEAK_TFLOPS = 157.3
+
+
+def cpu_baseline(p, ev, workers: int, photons_per_worker: int):
+    """Reference C branch (or the oracle) on the first events of the
+    workload; ~photons_per_worker photons per process."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+
+    kind = "reference" if po.ref_available() else "port"
+    cum = np.cumsum(ev["N"].astype(np.int64))
+    n_ev = int(np.searchsorted(cum, photons_per_worker * workers)) + 1
+    sample = ev[:n_ev]
+    photons = int(sample["N"].sum())
+    res, secs = po.run_parallel(p, sample, p["num_hist"], workers=workers,
+                                kind="ref" if kind == "reference" else "oracle")
+    wall = max(secs)
+    return {"value": photons / wall, "unit": "optical photons/s",
+            "cores": workers, "kind": kind,
+            "sample": f"first {n_ev} events ({photons} photons) of the workload, "
+                      f"{workers} processes over contiguous slices, "
+                      f"slowest worker {wall:.1f} s"}
+
+
+def flops_model(p, ev, n_photons: int = 60000):
+    import numpy as np
+
+    from hybridmantis_amd import flops
+    from oracle import pyoracle as po
+
+    cum = np.cumsum(ev["N"].astype(np.int64))
+    n_ev = int(np.searchsorted(cum, n_photons)) + 1
+    r = po.run(p, ev[:n_ev], p["num_hist"], math=po.MATH_DEVICE)
+    ph = float(r.counters[0])
+    return flops.flops_per_photon(r.work, ph), flops.per_photon(r.work, ph)
+
+
+def traffic_from_profile(name: str):
+    """HBM bytes per launch recorded by the rocprofv3 PMC runs in
+    profiles/<name> (line 'traffic_bytes_per_launch: N')."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-workers", type=int, default=0)
+    ap.add_argument("--cpu-photons", type=int, default=150000,
+                    help="photons per CPU process in the baseline sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(histories shard across GPUs with one tally all-reduce; not built). "
+              "Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    from hybridmantis_amd import events
+
+    p, ev = events.workload("mono25x50")
+    n_photons = int(ev["N"].sum())
+
+    # CPU legs first, before this process touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(p, ev, workers, args.cpu_photons)
+    fpp, per_photon = flops_model(p, ev)
+
+    from hybridmantis_amd import fd2
+
+    tr = fd2.Transport(p)
+    buf = tr.device_buffer(ev.nbytes)
+    buf.upload(ev)
+    for _ in range(args.warmup):
+        tr.submit_device(buf, len(ev))
+        tr.sync()
+    tr.reset()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tr.submit_device(buf, len(ev))
+        tr.sync()
+        kernel_ms.append(tr.last_kernel_ms())
+    wall = time.perf_counter() - t0
+    counters = tr.counters()
+    assert counters["generated"] == n_photons * args.steps
+    buf.free()
+    tr.close()
+
+    avg_ms = sum(kernel_ms) / len(kernel_ms)
+    achieved = fpp * n_photons / (avg_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": n_photons * args.steps / wall,
+        "unit": "optical photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "mono25x50: 100,000 histories at 25 keV, pencil beam, "
+                               "H=150 um, R=2.55 um, 501x501 PRF at 1 um, "
+                               f"{len(ev)} events, {n_photons} optical photons",
+                   "histories": p["num_hist"], "events": int(len(ev)),
+                   "photons": n_photons, "parallelism": "1 GPU"},
+        "roofline": {"bound": "fp32-valu", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic_from_profile("transport_mono25x50_pmc.md"),
+                     "kernel_ms": avg_ms, "flops_per_photon": fpp},
+        "cpu_baseline": cpu,
+        "histories_per_s": p["num_hist"] * args.steps / wall,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

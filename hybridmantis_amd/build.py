@@ -22,8 +22,8 @@ CXX_SOURCES = ["fd2_io.cpp", "fd2_fortran.cpp"]
 
 ARCH = os.environ.get("FD2_OFFLOAD_ARCH", "gfx950")
 
-# Packed-FP32 instructions are left to the compiler's own judgement off:
-# every HIP translation unit is built with the feature disabled.
+# Every HIP translation unit is built with the packed-FP32 target feature
+# disabled (no v_pk_{add,mul,fma}_f32).
 # -ffp-contract=off: no implicit FMA, so the transport's float expressions
 # round exactly like the CPU restatement of the same expressions.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -52,9 +52,10 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         if src in HIP_SOURCES:
             cmd = [hipcc, "-x", "hip", *HIP_FLAGS, "-c", path, "-o", obj]
-        else:
-            cmd = [hipcc, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I",
-                   os.path.join(ROOT, "include"), "-c", path, "-o", obj]
+        else:  # host-only C++ (file formats, Fortran-ABI shim)
+            cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC",
+                   "-Wall", "-I", os.path.join(ROOT, "include"), "-c", path,
+                   "-o", obj]
         cmds.append(cmd)
         objs.append(obj)
     procs = []

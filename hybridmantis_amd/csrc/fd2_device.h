@@ -312,29 +312,34 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
   else if (r2 > r1)
     dcyl = r2;
   float d;
+  int plane = 0;  // traced: the plane z = +-H/2 was the nearer surface
   if (p.uz > 0.0f) {
     const float dpl = (H2 - p.pz) / p.uz;
     if (fabsf(p.ux) < g.eps_lt && fabsf(p.uy) < g.eps_lt && fabsf(p.uz - 1.0f) < g.eps_lt) {
       d = dpl;
       p.pz = H2;
+      plane = 1;
     } else if (dpl >= dcyl) {
       d = dcyl;
       p.pz = t2 + d * p.uz;
     } else {
       d = dpl;
       p.pz = H2;
+      plane = 1;
     }
   } else if (p.uz < 0.0f) {
     const float dpl = (-H2 - p.pz) / p.uz;
     if (fabsf(p.ux) < g.eps_lt && fabsf(p.uy) < g.eps_lt && fabsf(p.uz + 1.0f) < g.eps_lt) {
       d = dpl;
       p.pz = -H2;
+      plane = 1;
     } else if (dpl >= dcyl) {
       d = dcyl;
       p.pz = t2 + d * p.uz;
     } else {
       d = dpl;
       p.pz = -H2;
+      plane = 1;
     }
   } else {
     d = dcyl;
@@ -344,6 +349,7 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
   p.py = t1 + d * p.uy;
 
   int code = 0;
+  uint32_t decisions = 0, iters = 0;  // traced nudge moves (1) / halvings (0)
   if (out_of_slab(p, g)) {
     code = 1;
   } else {
@@ -363,7 +369,9 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
         p.py = cy;
         rn = rc;
         moved++;
+        if (TRACE) decisions |= 1u << iters;
       }
+      if (TRACE) ++iters;
     }
     if (rn > g.r_nudge)
       code = 2;
@@ -380,7 +388,7 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
       o[8] = p.px; o[9] = p.py; o[10] = p.pz;
       o[11] = __int_as_float(code);
       o[12] = __int_as_float(sink->photon);
-      o[13] = 0.0f;
+      o[13] = __uint_as_float(decisions | (iters << 20) | ((uint32_t)plane << 25));
     }
   }
   if (code == 1) {

@@ -23,7 +23,7 @@ PHOTON_DTYPE = np.dtype([("fate", "<i4"), ("pix", "<i4"), ("x", "<f4"),
                          ("y", "<f4")])
 STEP_DTYPE = np.dtype([("pos_in", "<f4", 3), ("dcos", "<f4", 3), ("xc", "<f4"),
                        ("yc", "<f4"), ("pos_out", "<f4", 3), ("code", "<i4"),
-                       ("photon", "<i4"), ("reserved", "<i4")])
+                       ("photon", "<i4"), ("decisions", "<u4")])
 COUNTER_NAMES = ["generated", "detected", "abs_top", "abs_bulk", "lost",
                  "outofcol", "theta1", "regenerated"]
 

@@ -147,7 +147,10 @@ typedef struct fd2_step {
   float pos_in[3], dcos[3], xc, yc, pos_out[3];
   int32_t code;             /* 0 moved, 1 lost, 2 out of column              */
   int32_t photon;
-  int32_t reserved;
+  /* branch decisions of the step: bits 0-19 out-of-column nudge moves (1)
+   * or step halvings (0) in order, bits 20-24 their number, bit 25 the
+   * plane z=+-H/2 was nearer than the wall */
+  uint32_t decisions;
 } fd2_step;
 int fd2_debug_steps(fd2_ctx *ctx, const fd2_event *events, int64_t n,
                     int32_t max_photons, fd2_step *steps, int32_t max_steps,

@@ -969,9 +969,13 @@ int fdo_phs(const uint32_t *hist_count, int32_t num_hist, int min_ph,
 
 /* ------------------------------------------------------ FP64 re-trace */
 
-/* dist_to_surface (K:966-1185) of one step in double from a float state. */
+/* dist_to_surface (K:966-1185) of one step in double from a float state.
+ * forced >= 0 replays the FP32 run's branch decisions (layout of
+ * fd2_step.decisions: nudge moves/halvings and plane-vs-wall) instead of
+ * re-deciding them in double. */
 int fdo_retrace_step_f64(const float pin[3], const float dc[3], float xcf,
-                         float ycf, const fdo_geom *g, double po[3]) {
+                         float ycf, const fdo_geom *g, int64_t forced,
+                         double po[3]) {
   const double H = g->H, R = g->R, xc = xcf, yc = ycf;
   double p[3] = {pin[0], pin[1], pin[2]}, u[3] = {dc[0], dc[1], dc[2]};
   double qa = u[0] * u[0] + u[1] * u[1];
@@ -985,7 +989,17 @@ int fdo_retrace_step_f64(const float pin[3], const float dc[3], float xcf,
   else if (r2 > r1)
     dcyl = r2;
   double d, t2 = p[2];
-  if (u[2] > 0.0) {
+  if (forced >= 0 && u[2] != 0.0) {
+    const int plane = (int)((forced >> 25) & 1);
+    const double zp = u[2] > 0.0 ? H / 2 : -H / 2;
+    if (plane) {
+      d = (zp - p[2]) / u[2];
+      p[2] = zp;
+    } else {
+      d = dcyl;
+      p[2] = t2 + d * u[2];
+    }
+  } else if (u[2] > 0.0) {
     if (fabs(u[0]) < FDO_EPS && fabs(u[1]) < FDO_EPS && fabs(u[2] - 1.0) < FDO_EPS) {
       d = (H / 2 - p[2]) / u[2];
       p[2] = H / 2;
@@ -1015,6 +1029,19 @@ int fdo_retrace_step_f64(const float pin[3], const float dc[3], float xcf,
   double rn = sqrt((p[0] - xc) * (p[0] - xc) + (p[1] - yc) * (p[1] - yc));
   int moved = 0, shrunk = 0;
   double step = 0.5;
+  if (forced >= 0) {
+    const int iters = (int)((forced >> 20) & 31);
+    for (int it = 0; it < iters; ++it) {
+      if ((forced >> it) & 1) {
+        p[0] -= step * u[0];
+        p[1] -= step * u[1];
+      } else {
+        step /= 2;
+      }
+    }
+    memcpy(po, p, sizeof p);
+    return 0;
+  }
   while (rn > R - 1e-5 && moved < 10 && shrunk < 10) {
     double s0 = p[0], s1 = p[1];
     p[0] -= step * u[0];
@@ -1028,4 +1055,87 @@ int fdo_retrace_step_f64(const float pin[3], const float dc[3], float xcf,
   if (OUT64(p)) return 1;
 #undef OUT64
   return 0;
+}
+
+/* ------------------------------------------------------- test hooks */
+
+/* n uniforms of photon `photon`'s counter-mode stream */
+void fdo_counter_stream(uint32_t seed, uint32_t event, uint32_t photon,
+                        int64_t n, float *out) {
+  fdo_tally t;
+  memset(&t, 0, sizeof t);
+  ctx c;
+  memset(&c, 0, sizeof c);
+  c.t = &t;
+  c.rng_mode = FDO_RNG_COUNTER;
+  c.key[0] = seed;
+  c.key[1] = event;
+  counter_photon(&c, photon);
+  for (int64_t i = 0; i < n; ++i) out[i] = uni(&c);
+}
+
+/* fd2_math.h evaluated on the CPU: fn 0 sin, 1 cos, 2 log, 3 asin, 4 acos */
+int fdo_math_eval(int fn, const float *x, float *y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    switch (fn) {
+      case 0: y[i] = fd2_sin(x[i]); break;
+      case 1: y[i] = fd2_cos(x[i]); break;
+      case 2: y[i] = fd2_log(x[i]); break;
+      case 3: y[i] = fd2_asin(x[i]); break;
+      case 4: y[i] = fd2_acos(x[i]); break;
+      default: return -1;
+    }
+  }
+  return 0;
+}
+
+/* Fresnel probabilities (K:1811-1871 formula) for given angles */
+void fdo_fresnel_probs(float t1, float t2, int math_mode, float *pr, float *pt) {
+  ctx c;
+  memset(&c, 0, sizeof c);
+  c.dev = math_mode == FDO_MATH_DEVICE;
+  fresnel_pq(&c, t1, t2, pr, pt);
+}
+
+/* n emission directions (K:578-592) from counter-mode streams of event 0,
+ * photons 1..n (photon 0 would use the first-photon rejection) */
+void fdo_emit_dirs(uint32_t seed, int math_mode, int64_t n, float *out3) {
+  fdo_tally t;
+  memset(&t, 0, sizeof t);
+  ctx c;
+  memset(&c, 0, sizeof c);
+  c.t = &t;
+  c.rng_mode = FDO_RNG_COUNTER;
+  c.dev = math_mode == FDO_MATH_DEVICE;
+  c.key[0] = seed;
+  c.key[1] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    counter_photon(&c, (uint32_t)(i + 1));
+    emit_dir(&c);
+    out3[3 * i] = c.dcos[0];
+    out3[3 * i + 1] = c.dcos[1];
+    out3[3 * i + 2] = c.dcos[2];
+  }
+}
+
+/* n RoughSurface perturbations (K:3261-3300) of the normal (nx,ny,nz) */
+void fdo_rough_normals(uint32_t seed, int math_mode, float beta, const float n0[3],
+                       int64_t n, float *out3) {
+  fdo_tally t;
+  memset(&t, 0, sizeof t);
+  ctx c;
+  memset(&c, 0, sizeof c);
+  c.t = &t;
+  c.rng_mode = FDO_RNG_COUNTER;
+  c.dev = math_mode == FDO_MATH_DEVICE;
+  c.key[0] = seed;
+  c.key[1] = 1;
+  for (int64_t i = 0; i < n; ++i) {
+    counter_photon(&c, (uint32_t)i);
+    float v[3] = {n0[0], n0[1], n0[2]};
+    rough(&c, v, beta);
+    out3[3 * i] = v[0];
+    out3[3 * i + 1] = v[1];
+    out3[3 * i + 2] = v[2];
+  }
 }

@@ -131,9 +131,20 @@ int fdo_phs(const uint32_t *hist_count, int32_t num_hist, int min_ph,
 
 /* FP64 re-trace of one in-column step (dist_to_surface) from a float state:
  * the position the step reaches, computed in double.  Returns the reference
- * code: 0 moved, 1 lost, 2 out of column. */
+ * code: 0 moved, 1 lost, 2 out of column.  forced >= 0 replays recorded
+ * branch decisions (fd2_step.decisions layout; the code is then 0). */
 int fdo_retrace_step_f64(const float pos_in[3], const float dcos[3], float xc,
-                         float yc, const fdo_geom *g, double pos_out[3]);
+                         float yc, const fdo_geom *g, int64_t forced,
+                         double pos_out[3]);
+
+/* Test hooks (closed-form and distribution checks). */
+void fdo_counter_stream(uint32_t seed, uint32_t event, uint32_t photon,
+                        int64_t n, float *out);
+int fdo_math_eval(int fn, const float *x, float *y, int64_t n);
+void fdo_fresnel_probs(float t1, float t2, int math_mode, float *pr, float *pt);
+void fdo_emit_dirs(uint32_t seed, int math_mode, int64_t n, float *out3);
+void fdo_rough_normals(uint32_t seed, int math_mode, float beta,
+                       const float n0[3], int64_t n, float *out3);
 
 #ifdef __cplusplus
 }

@@ -90,7 +90,7 @@ def lib():
         _lib.fdo_phs.restype = C.c_int
         _lib.fdo_retrace_step_f64.argtypes = [
             C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float,
-            C.POINTER(Geom), C.POINTER(C.c_double)]
+            C.POINTER(Geom), C.c_int64, C.POINTER(C.c_double)]
         _lib.fdo_retrace_step_f64.restype = C.c_int
     return _lib
 
@@ -239,10 +239,106 @@ def philox(ctr, key):
     return [int(v) for v in o]
 
 
-def retrace_step(pos, dcos, xc, yc, p: dict):
+def counter_stream(seed: int, event: int, photon: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    f = lib().fdo_counter_stream
+    f.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int64, C.c_void_p]
+    f(seed & 0xFFFFFFFF, event & 0xFFFFFFFF, photon & 0xFFFFFFFF, n, out.ctypes.data)
+    return out
+
+
+MATH_FN = {"sin": 0, "cos": 1, "log": 2, "asin": 3, "acos": 4}
+
+
+def math_eval(fn: str, x) -> np.ndarray:
+    xs = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(xs)
+    f = lib().fdo_math_eval
+    f.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
+    f.restype = C.c_int
+    if f(MATH_FN[fn], xs.ctypes.data, y.ctypes.data, len(xs)) != 0:
+        raise ValueError(fn)
+    return y
+
+
+def fresnel_probs(t1: float, t2: float, math: int = MATH_FAITHFUL):
+    pr, pt = C.c_float(), C.c_float()
+    f = lib().fdo_fresnel_probs
+    f.argtypes = [C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float),
+                  C.POINTER(C.c_float)]
+    f(t1, t2, math, C.byref(pr), C.byref(pt))
+    return pr.value, pt.value
+
+
+def emit_dirs(seed: int, n: int, math: int = MATH_FAITHFUL) -> np.ndarray:
+    out = np.zeros((n, 3), np.float32)
+    f = lib().fdo_emit_dirs
+    f.argtypes = [C.c_uint32, C.c_int, C.c_int64, C.c_void_p]
+    f(seed & 0xFFFFFFFF, math, n, out.ctypes.data)
+    return out
+
+
+def rough_normals(seed: int, beta: float, n0, n: int,
+                  math: int = MATH_FAITHFUL) -> np.ndarray:
+    out = np.zeros((n, 3), np.float32)
+    v = (C.c_float * 3)(*n0)
+    f = lib().fdo_rough_normals
+    f.argtypes = [C.c_uint32, C.c_int, C.c_float, C.c_void_p, C.c_int64, C.c_void_p]
+    f(seed & 0xFFFFFFFF, math, beta, v, n, out.ctypes.data)
+    return out
+
+
+def retrace_step(pos, dcos, xc, yc, p: dict, forced: int = -1):
+    """FP64 re-trace of one in-column step; forced = recorded decisions
+    (fd2_step.decisions) to follow the FP32 run's branches."""
     g = make_geom(p)
-    pi = (C.c_float * 3)(*pos)
-    di = (C.c_float * 3)(*dcos)
+    pi = (C.c_float * 3)(*[float(v) for v in pos])
+    di = (C.c_float * 3)(*[float(v) for v in dcos])
     po = (C.c_double * 3)()
-    code = lib().fdo_retrace_step_f64(pi, di, xc, yc, C.byref(g), po)
+    code = lib().fdo_retrace_step_f64(pi, di, float(xc), float(yc), C.byref(g),
+                                      int(forced), po)
     return code, np.array(po[:])
+
+
+# ---------------------------------------------------------- multi-process
+
+def _slice_worker(args):
+    kind, p, ev, num_hist, rng, seed, base, math, rezero = args
+    import time as _t
+    t0 = _t.perf_counter()
+    if kind == "ref":
+        r = run_ref(p, ev, num_hist, seed=seed, event_base=base, rezero=rezero)
+        work = None
+    else:
+        r = run(p, ev, num_hist, rng=rng, seed=seed, event_base=base, math=math)
+        work = r.work
+    return (r.prf, r.hist_count, r.counters, work, _t.perf_counter() - t0)
+
+
+def run_parallel(p: dict, events, num_hist: int, workers: int = 8,
+                 kind: str = "oracle", rng: int = RNG_COUNTER,
+                 seed: int = 271828182, math: int = MATH_FAITHFUL,
+                 rezero: bool = False, timeout: float = 600.0):
+    """Run contiguous event slices in separate processes (the transport
+    keeps per-process state, like the reference C path) and sum the
+    tallies; results equal a single-process run because streams are keyed
+    by the global event number.  Returns (Result, per-worker seconds)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+
+    ev = _as_events(events)
+    n = len(ev)
+    workers = max(1, min(workers, n))
+    bounds = [n * i // workers for i in range(workers + 1)]
+    jobs = [(kind, p, ev[bounds[i]:bounds[i + 1]].copy(), num_hist, rng, seed,
+             bounds[i], math, rezero) for i in range(workers)]
+    # fresh interpreters (spawn): the caller may hold a GPU context; a dead
+    # worker raises BrokenProcessPool instead of hanging
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+        futs = [ex.submit(_slice_worker, j) for j in jobs]
+        outs = [f.result(timeout=timeout) for f in futs]
+    prf = sum(o[0] for o in outs)
+    hc = sum(o[1].astype(np.uint64) for o in outs).astype(np.uint32)
+    cnt = sum(o[2] for o in outs)
+    work = None if outs[0][3] is None else sum(o[3] for o in outs)
+    return Result(prf, hc, cnt, work, None), [o[4] for o in outs]

@@ -1,0 +1,226 @@
+"""GPU transport (libfd2, HIP on gfx950) against the oracle.
+
+* Trajectory tests: the GPU and the oracle's FP32 restatement
+  (MATH_DEVICE) consume the same uniform streams -- the reference's
+  per-event RANECU streams (one lane per event) and the production kernel's
+  per-photon counter streams -- and must give every photon the same fate
+  and PRF pixel (at most 0.1% of photons may differ; a divergence in a
+  RANECU event shifts the rest of that event's stream, so only the first
+  differing photon of an event is counted).  The oracle's control flow is
+  itself pinned bit-exactly to the reference's C branch
+  (tests/test_oracle_golden.py).
+* Statistical test: the production kernel against the oracle's faithful
+  mode (the C path's float/double arithmetic) on the same events: PRF and
+  PHS chi-square p > 0.01, mean detected photons and Swank factor within
+  0.5%.
+* Determinism, split invariance, FP64 re-trace of positions, edge cases and
+  size-independent properties at the full 1.1e8-photon workload.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from hybridmantis_amd import events as E
+from hybridmantis_amd import stats
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["mono25_small", "example_input", "smooth_nobulk_notop",
+         "low_index_sensor_refl", "detector_edges", "thick_50kev"]
+
+
+def load_case(golden, name):
+    here, man = golden
+    m = man[name]
+    d = np.load(os.path.join(here, m["file"]))
+    return m["params"], d["events"], m["seed"]
+
+
+def first_divergences(events, a, b):
+    """Photons compared up to and including each event's first differing
+    photon; returns (differing events, photons compared)."""
+    n = np.maximum(events["N"], 0)
+    off = np.concatenate([[0], np.cumsum(n)])
+    bad, compared = 0, 0
+    diff = (a["fate"] != b["fate"]) | (a["pix"] != b["pix"])
+    for e in range(len(n)):
+        d = diff[off[e]:off[e + 1]]
+        if d.any():
+            bad += 1
+            compared += int(np.argmax(d)) + 1
+        else:
+            compared += int(n[e])
+    return bad, compared
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_trajectory_ranecu_matches_oracle(fd2lib, oracle, golden, name):
+    p, ev, seed = load_case(golden, name)
+    print(f"[trajectory-ranecu] {name}: {len(ev)} events, {int(ev['N'].sum())} photons",
+          flush=True)
+    with fd2lib.Transport(p) as tr:
+        rec = tr.trace_ranecu(ev, seed_input=seed)
+        counters = tr.counters()
+        prf, hist = tr.prf(), tr.hist_counts()
+    o = oracle.run(p, ev, p["num_hist"], rng=oracle.RNG_RANECU, seed=seed,
+                   record=True, math=oracle.MATH_DEVICE)
+    bad, compared = first_divergences(ev, rec, o.photons)
+    assert bad <= 0.001 * compared, (bad, compared)
+    if bad == 0:
+        assert np.array_equal(prf, o.prf)
+        assert np.array_equal(hist, o.hist_count)
+        assert [counters[k] for k in oracle.COUNTER_NAMES] == o.counters.tolist()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_production_kernel_matches_oracle_per_photon(fd2lib, oracle, golden, name):
+    p, ev, seed = load_case(golden, name)
+    print(f"[trajectory-counter] {name}", flush=True)
+    with fd2lib.Transport(p, seed=seed) as tr:
+        rec = tr.submit_record(ev)
+        prf, hist, counters = tr.prf(), tr.hist_counts(), tr.counters()
+    o = oracle.run(p, ev, p["num_hist"], rng=oracle.RNG_COUNTER, seed=seed,
+                   record=True, math=oracle.MATH_DEVICE)
+    diff = (rec["fate"] != o.photons["fate"]) | (rec["pix"] != o.photons["pix"])
+    assert diff.sum() <= 0.001 * len(rec), diff.sum()
+    if not diff.any():
+        assert np.array_equal(prf, o.prf)
+        assert np.array_equal(hist, o.hist_count)
+        assert [counters[k] for k in oracle.COUNTER_NAMES] == o.counters.tolist()
+    # detection points: identical FP32 values
+    det = (rec["fate"] == 1) & ~diff
+    assert np.array_equal(rec["x"][det], o.photons["x"][det])
+
+
+def test_statistics_match_faithful_oracle(fd2lib, oracle):
+    """500 histories at 25 keV (~5.5e5 photons), same events and streams;
+    GPU in FP32, oracle in the C path's float/double arithmetic."""
+    p, ev = E.workload("mono25", scale=0.25)
+    nh = p["num_hist"]
+    print(f"[statistics] {len(ev)} events {int(ev['N'].sum())} photons", flush=True)
+    with fd2lib.Transport(p) as tr:
+        tr.submit(ev)
+        g_prf, g_hist, g_phs, g_cnt = tr.prf(), tr.hist_counts(), tr.phs(), tr.counters()
+    o, _ = oracle.run_parallel(p, ev, nh, workers=min(8, os.cpu_count() or 1),
+                               math=oracle.MATH_FAITHFUL)
+    o_phs = oracle.phs(o.hist_count, p["min_ph"], p["max_ph"], p["num_bins"])
+    # PHS in 40-photon bins
+    _, _, p_phs = stats.chi2_two_sample(g_phs.reshape(-1, 40).sum(1),
+                                        o_phs.reshape(-1, 40).sum(1))
+    assert p_phs > 0.01
+    # PRF: radial profile about the beam (pixel 250.5,250.5), 1-um annuli
+    edges = np.concatenate([np.arange(0, 20, 1.0), np.geomspace(20, 250, 30)])
+    rg = stats.radial_profile(g_prf, (250.5, 250.5), edges)
+    ro = stats.radial_profile(o.prf, (250.5, 250.5), edges)
+    _, _, p_prf = stats.chi2_two_sample(rg, ro)
+    assert p_prf > 0.01
+    assert abs(stats.mean_detected(g_hist) / stats.mean_detected(o.hist_count) - 1) < 0.005
+    assert abs(stats.swank(g_hist) / stats.swank(o.hist_count) - 1) < 0.005
+    assert abs(g_cnt["detected"] / o.counter("detected") - 1) < 0.005
+    assert g_cnt["generated"] == o.counter("generated")
+
+
+def test_deterministic_and_split_invariant(fd2lib):
+    p, ev = E.workload("mono25", scale=0.2)
+    with fd2lib.Transport(p) as tr:
+        tr.submit(ev)
+        a = (tr.prf(), tr.hist_counts(), tr.counters())
+        tr.reset()
+        tr.submit(ev)
+        b = (tr.prf(), tr.hist_counts(), tr.counters())
+        tr.reset()
+        cut = [0, 7, len(ev) // 2, len(ev)]
+        for i in range(3):
+            tr.submit(ev[cut[i]:cut[i + 1]])
+        c = (tr.prf(), tr.hist_counts(), tr.counters())
+        tr.reset()
+        tr.set_tuning(chunk=16, waves_per_cu=8)
+        tr.submit(ev)
+        d = (tr.prf(), tr.hist_counts(), tr.counters())
+    for x in (b, c, d):
+        assert np.array_equal(a[0], x[0])
+        assert np.array_equal(a[1], x[1])
+        assert a[2] == x[2]
+
+
+def test_fp64_retrace_of_positions(fd2lib, oracle):
+    """Every in-column step of 64 photons: the FP32 position reached equals
+    the FP64 re-trace from the same FP32 state within 1e-5 relative."""
+    p, ev = E.workload("mono25", scale=0.02)
+    big = ev[np.argmax(ev["N"])]
+    with fd2lib.Transport(p) as tr:
+        steps = tr.debug_steps(np.array([big]), 64, 200000)
+    assert len(steps) > 1000
+    worst = 0.0
+    for s in steps:
+        if s["code"] != 0:
+            continue  # lost / out of column: no end point to compare
+        # forced along the FP32 run's decisions (plane vs wall, each nudge)
+        _, pos = oracle.retrace_step(s["pos_in"], s["dcos"], s["xc"], s["yc"], p,
+                                     forced=int(s["decisions"]))
+        rel = np.abs(pos - s["pos_out"]) / np.maximum(np.abs(pos), 1.0)
+        worst = max(worst, float(rel.max()))
+    assert worst < 1e-5, worst
+
+
+def test_edge_cases(fd2lib, oracle):
+    p = E.params(num_hist=5)
+    with fd2lib.Transport(p) as tr:
+        tr.submit(np.zeros(0, E.EVENT_DTYPE))  # empty buffer
+        ev = np.zeros(4, E.EVENT_DTYPE)
+        ev["x"], ev["y"], ev["z"], ev["hist"] = 500.5, 500.5, 10.0, [1, 2, 3, 4]
+        ev["N"] = [0, 0, 1, 0]  # ragged: events without photons
+        rec = tr.submit_record(ev)
+        assert len(rec) == 1 and rec["fate"][0] != 0
+        assert tr.counters()["generated"] == 1
+        big = ev[:1].copy()
+        big["N"] = 900000  # the reference's max_photon_per_EDE
+        tr.reset()
+        tr.submit(big)
+        c = tr.counters()
+        assert c["generated"] == 900000
+        assert c["generated"] == sum(c[k] for k in ("detected", "abs_top", "abs_bulk",
+                                                    "lost", "outofcol", "theta1"))
+        assert tr.hist_counts()[0] == c["detected"]
+        big["N"] = 900001
+        with pytest.raises(fd2lib.Fd2Error, match="900000"):
+            tr.submit(big)
+        bad = ev[:1].copy()
+        bad["N"], bad["geom"] = 5, 3
+        with pytest.raises(fd2lib.Fd2Error):
+            tr.submit(bad)
+        # an event on the detector's edge: its photons leave the detector
+        edge = ev[:1].copy()
+        edge["x"], edge["N"] = 0.5, 2000
+        tr.reset()
+        rec = tr.submit_record(edge)
+        o = oracle.run(p, edge, 5, rng=oracle.RNG_COUNTER, math=oracle.MATH_DEVICE,
+                       record=True, event_base=0)
+        assert (rec["fate"] == 5).mean() > 0.3
+        assert np.array_equal(rec["fate"], o.photons["fate"])
+
+
+def test_full_workload_properties(fd2lib):
+    """100,000 histories (1.1e8 photons): conservation of photons and
+    detections, and bit-identical tallies on a second run."""
+    p, ev = E.workload("mono25x50")
+    n = int(ev["N"].sum())
+    print(f"[full] {len(ev)} events {n} photons", flush=True)
+    with fd2lib.Transport(p) as tr:
+        tr.submit(ev)
+        prf, hist, c = tr.prf(), tr.hist_counts(), tr.counters()
+        ms = tr.last_kernel_ms()
+        tr.reset()
+        tr.submit(ev)
+        prf2, hist2, c2 = tr.prf(), tr.hist_counts(), tr.counters()
+    print(f"[full] kernel {ms:.1f} ms, {n / ms * 1e3:.3e} photons/s", flush=True)
+    assert c["generated"] == n
+    assert n == sum(c[k] for k in ("detected", "abs_top", "abs_bulk", "lost",
+                                   "outofcol", "theta1"))
+    assert int(hist.sum()) == c["detected"]
+    assert int(prf.sum()) <= c["detected"]
+    assert np.array_equal(prf, prf2) and np.array_equal(hist, hist2) and c == c2
+    # the PRF peaks at the beam pixel (500.5 um -> pixel 250)
+    i, j = np.unravel_index(np.argmax(prf), prf.shape)
+    assert abs(i - 250) <= 2 and abs(j - 250) <= 2

@@ -1,0 +1,77 @@
+"""The oracle against the reference's own C branch.
+
+Fixtures (tests/golden/ref_*.npz) hold what main/kernel_cuda_c_ver1_0.cu's C
+branch produced for fixed events and seeds (tests/golden/make_golden.py):
+every photon's fate and PRF pixel, the PRF, the per-history counts and the
+counters.  The oracle in its faithful mode must reproduce them bit for bit.
+Where oracle/_ref is built (the build container) a live comparison on fresh
+events runs as well.
+"""
+import os
+
+import numpy as np
+import pytest
+
+CASES = ["mono25_small", "example_input", "smooth_nobulk_notop",
+         "low_index_sensor_refl", "detector_edges", "thick_50kev"]
+
+
+def load(golden, name):
+    here, man = golden
+    m = man[name]
+    d = np.load(os.path.join(here, m["file"]))
+    prf = np.zeros(int(np.prod(d["prf_shape"])), np.uint64)
+    prf[d["prf_idx"]] = d["prf_val"]
+    return m, d, prf.reshape(tuple(d["prf_shape"]))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_matches_reference_fixture(oracle, golden, name):
+    m, d, prf = load(golden, name)
+    p = m["params"]
+    r = oracle.run(p, d["events"], p["num_hist"], rng=oracle.RNG_RANECU,
+                   seed=m["seed"], record=True, math=oracle.MATH_FAITHFUL)
+    assert np.array_equal(r.photons["fate"], d["fate"].astype(np.int32))
+    assert np.array_equal(r.photons["pix"], d["pix"])
+    assert np.array_equal(r.prf, prf)
+    assert np.array_equal(r.hist_count, d["hist"])
+    assert np.array_equal(r.counters[:7], d["counters"][:7])
+
+
+def test_fixture_counters_are_consistent(golden):
+    # each photon ends exactly once: generated = sum of the fate counters
+    for name in CASES:
+        _, d, prf = load(golden, name)
+        c = d["counters"]
+        assert c[0] == c[1:7].sum()
+        assert c[1] == (d["fate"] == 1).sum() + (d["fate"] == 2).sum()
+        assert prf.sum() == (d["fate"] == 1).sum()
+        assert d["hist"].sum() == c[1]
+
+
+def test_device_math_mode_agrees_statistically(oracle, golden):
+    # same streams, FP32 arithmetic: fates may differ photon by photon
+    # (rounding-level chaos) but the tallies agree within counting noise
+    m, d, _ = load(golden, "mono25_small")
+    p = m["params"]
+    a = oracle.run(p, d["events"], p["num_hist"], math=oracle.MATH_FAITHFUL)
+    b = oracle.run(p, d["events"], p["num_hist"], math=oracle.MATH_DEVICE)
+    n = a.counter("generated")
+    for k in ("detected", "abs_top", "abs_bulk", "lost"):
+        x, y = a.counter(k), b.counter(k)
+        assert abs(x - y) <= 5 * np.sqrt(max(x, 1)) + 5, (k, x, y, n)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(
+    os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+    "libfdref.so")), reason="reference harness not built here")
+def test_oracle_matches_reference_live(oracle):
+    from hybridmantis_amd import events as E
+
+    p = E.params(num_hist=30, R=3.0, H=120.0, sensor_refl=0.1, beta=0.35)
+    ev = E.generate(30, p["H"], 18.0, 100.0, 99)
+    r = oracle.run_ref(p, ev, 30, seed=987654321, record=True)
+    o = oracle.run(p, ev, 30, rng=oracle.RNG_RANECU, seed=987654321, record=True)
+    assert np.array_equal(r.photons["fate"], o.photons["fate"])
+    assert np.array_equal(r.photons["pix"], o.photons["pix"])
+    assert np.array_equal(r.prf, o.prf)

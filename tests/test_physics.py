@@ -1,0 +1,103 @@
+"""Closed-form checks of the transport's building blocks (oracle)."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from hybridmantis_amd import events as E
+
+
+@pytest.mark.parametrize("math", [0, 1])
+def test_fresnel_energy_conservation_and_normal_incidence(oracle, math):
+    # Rs + Ts = 1 and Rp + Tp = 1 for Snell-related angles, so the
+    # reference's Pr + Pt = 1 before its normalisation
+    rng = np.random.default_rng(1)
+    for n12 in (1.8, 1 / 1.8):
+        tmax = np.arcsin(min(1.0, 1.0 / n12)) * 0.999
+        for t1 in rng.uniform(0.01, tmax, 200):
+            t2 = np.arcsin(n12 * np.sin(t1))
+            pr, pt = oracle.fresnel_probs(np.float32(t1), np.float32(t2), math)
+            assert abs(pr + pt - 1.0) < 2e-4
+    # near normal incidence Pr -> ((n1-n2)/(n1+n2))^2
+    t1 = np.float32(0.00042)
+    t2 = np.float32(np.arcsin(1.8 * np.sin(0.00042)))
+    pr, pt = oracle.fresnel_probs(t1, t2, math)
+    assert abs(pr - (0.8 / 2.8) ** 2) < 1e-4
+
+
+@pytest.mark.parametrize("math", [0, 1])
+def test_emission_is_isotropic(oracle, math):
+    d = oracle.emit_dirs(3, 200000, math).astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    assert stats.kstest(d[:, 2], "uniform", args=(-1, 2)).pvalue > 1e-3
+    phi = np.arctan2(d[:, 1], d[:, 0])
+    assert stats.kstest(phi, "uniform", args=(-np.pi, 2 * np.pi)).pvalue > 1e-3
+
+
+@pytest.mark.parametrize("math", [0, 1])
+def test_rough_surface_cone(oracle, math):
+    n0 = np.array([1.0, 0.0, 0.0], np.float32)
+    for beta in (0.05, 0.2, 0.5):
+        v = oracle.rough_normals(11, beta, n0, 50000, math).astype(np.float64)
+        assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-6)
+        # normalise(n + beta p) with |p| = 1 lies within asin(beta) of n
+        ang = np.arccos(np.clip(v @ n0, -1, 1))
+        assert ang.max() <= np.arcsin(beta) + 1e-5
+        # the perturbation is symmetric about n in y
+        assert abs(v[:, 1].mean()) < 5 * v[:, 1].std() / np.sqrt(len(v))
+    v = oracle.rough_normals(11, 0.0, np.array([0.6, 0.8, 0.0], np.float32), 100, math)
+    assert np.allclose(v, [0.6, 0.8, 0.0], atol=1e-7)
+
+
+def test_step_retrace_closed_form(oracle):
+    """FP64 re-trace of an in-column step = ray/cylinder hit, then the
+    reference's nudge of 0.5 um back along -d (xy) off the wall."""
+    p = E.params()
+    rng = np.random.default_rng(2)
+    R, H = float(np.float32(p["R"])), float(np.float32(p["H"]))
+    checked = 0
+    for _ in range(300):
+        c = np.array([500.0, 500.0])
+        rr = R * np.sqrt(rng.uniform(0, 0.8))
+        a = rng.uniform(0, 2 * np.pi)
+        pos = np.array([c[0] + rr * np.cos(a), c[1] + rr * np.sin(a), rng.uniform(-40, 40)])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        pos32, d32 = pos.astype(np.float32), d.astype(np.float32)
+        P, D = pos32.astype(np.float64), d32.astype(np.float64)
+        qa = D[0] ** 2 + D[1] ** 2
+        qb = 2 * ((P[0] - c[0]) * D[0] + (P[1] - c[1]) * D[1])
+        qc = (P[0] - c[0]) ** 2 + (P[1] - c[1]) ** 2 - R * R
+        t_cyl = (-qb + np.sqrt(qb * qb - 4 * qa * qc)) / (2 * qa)
+        t_pl = ((H / 2 if D[2] > 0 else -H / 2) - P[2]) / D[2]
+        if t_pl < t_cyl:
+            continue  # plane hit: no wall
+        hit = P + t_cyl * D
+        want = hit.copy()
+        want[:2] -= 0.5 * D[:2]
+        if np.hypot(want[0] - c[0], want[1] - c[1]) > R - 1e-5:
+            continue  # grazing: the nudge would be repeated
+        code, got = oracle.retrace_step(pos32, d32, np.float32(c[0]), np.float32(c[1]), p)
+        assert code == 0
+        assert np.allclose(got, want, rtol=0, atol=1e-9)
+        checked += 1
+    assert checked > 100
+
+
+def test_phs_rule_examples(oracle):
+    # min 0, max 1400, 140 bins (the reference's example input): bin = c//10
+    counts = [0, 1, 9, 10, 15, 1399, 1400, 5000, 700]
+    h = oracle.phs(counts, 0, 1400, 140)
+    want = np.zeros(140, np.int32)
+    want[0], want[1], want[139], want[70] = 2, 2, 3, 1
+    assert np.array_equal(h, want)
+    # min 100: bin size (1400-100)//100 = 13, offset 100//13 = 7
+    h = oracle.phs([50, 100, 101, 130, 1399, 1400], 100, 1400, 100)
+    want = np.zeros(100, np.int32)
+    want[0] += 2            # <= min
+    want[101 // 13 - 7] += 1
+    want[130 // 13 - 7] += 1
+    want[99] += 1            # 1399//13 - 7 = 100 -> clamped to the last bin
+    want[99] += 1            # >= max
+    assert np.array_equal(h, want)
+    with pytest.raises(ValueError):
+        oracle.phs([1, 2], 0, 10, 20)  # bin size 0

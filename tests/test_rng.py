@@ -1,0 +1,88 @@
+"""Random-number generators: RANECU and its jump-ahead against the closed
+form and the reference's own routines; Philox4x32-10 against the published
+known-answer vectors; the per-photon streams' distribution."""
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+M1, A1, M2, A2 = 2147483563, 40014, 2147483399, 40692
+
+
+def ranecu_closed_form(s1, s2, n):
+    """L'Ecuyer (1988) combined MLCG from its definition s <- a s mod m."""
+    out = []
+    for _ in range(n):
+        s1 = (A1 * s1) % M1
+        s2 = (A2 * s2) % M2
+        z = s1 - s2
+        if z < 1:
+            z += M1 - 1
+        out.append(np.float32(z) * np.float32(1.0 / M1))
+    return np.array(out, np.float32)
+
+
+def test_ranecu_matches_closed_form(oracle):
+    for seeds in [(12345, 67890), (1, 1), (M1 - 1, M2 - 1), (2**30, 7)]:
+        got = oracle.ranecu_stream(seeds, 2000)
+        assert np.array_equal(got, ranecu_closed_form(*seeds, 2000))
+
+
+def test_init_prng_is_modular_jump(oracle):
+    # seed_k = seed_input * a_k^((batch+1)*50000*1000) mod m_k
+    for batch in (0, 1, 5, 2303999, 99999999):
+        for seed in (271828182, 1, 2000000000):
+            leap = (batch + 1) * 50000 * 1000
+            want = (seed * pow(A1, leap, M1) % M1, seed * pow(A2, leap, M2) % M2)
+            assert oracle.init_prng(batch, 50000, seed) == want
+
+
+def test_rng_matches_reference_fixture(oracle, golden):
+    here, man = golden
+    d = np.load(os.path.join(here, man["_rng"]["file"]))
+    for seed, batch, s0, s1 in d["init"]:
+        assert oracle.init_prng(int(batch), 50000, int(seed)) == (s0, s1)
+    for (seed, batch, s0, s1), ref_stream in zip(d["init"], d["streams"]):
+        assert np.array_equal(oracle.ranecu_stream((int(s0), int(s1)), 1000), ref_stream)
+    lib = oracle.lib()
+    for m, a, s, want in d["abmodm"]:
+        assert lib.fdo_abmodm(int(m), int(a), int(s)) == want
+        assert want == (int(a) * int(s)) % int(m)
+
+
+def test_ranecu_range_is_open_unit_interval():
+    # z in [1, 2147483562]: float(z) * float(1/m1) spans [4.66e-10, 0.99999994],
+    # so -ln(u) is finite and u < 1 always
+    scale = np.float32(1.0 / M1)
+    assert np.float32(1) * scale > 0
+    assert np.float32(2147483562) * scale == np.nextafter(np.float32(1), np.float32(0))
+
+
+# Random123 known-answer vectors for Philox4x32-10
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", KAT)
+def test_philox_known_answers(oracle, ctr, key, want):
+    assert oracle.philox(ctr, key) == want
+
+
+def test_counter_streams_uniform_and_independent(oracle):
+    u = np.concatenate([oracle.counter_stream(7, e, k, 500)
+                        for e in range(20) for k in range(100)])
+    assert u.min() > 0.0 and u.max() <= 1.0
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    # lag-1 serial correlation and cross-photon correlation near zero
+    assert abs(np.corrcoef(u[:-1], u[1:])[0, 1]) < 0.01
+    a = oracle.counter_stream(7, 3, 10, 20000)
+    b = oracle.counter_stream(7, 3, 11, 20000)
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.03
+    assert not np.array_equal(a[:16], b[:16])
+    # 24-bit resolution: values are multiples of 2^-24
+    assert np.all((u * 2**24) == np.round(u * 2**24))
