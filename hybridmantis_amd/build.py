@@ -40,9 +40,13 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libfd2")
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
+def build(verbose: bool = False, jobs: int = 4, out: str | None = None,
+          defines: tuple = ()) -> str:
+    """Compile and link libfd2.so (or `out`, with extra -D `defines`)."""
     hipcc = _hipcc()
-    objdir = os.path.join(HERE, "_build")
+    lib = out or LIB
+    objdir = os.path.join(HERE, "_build" if out is None else "_build_" +
+                          os.path.splitext(os.path.basename(out))[0])
     os.makedirs(objdir, exist_ok=True)
     cmds, objs = [], []
     for src in HIP_SOURCES + CXX_SOURCES:
@@ -51,7 +55,8 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
             continue
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         if src in HIP_SOURCES:
-            cmd = [hipcc, "-x", "hip", *HIP_FLAGS, "-c", path, "-o", obj]
+            cmd = [hipcc, "-x", "hip", *HIP_FLAGS, *[f"-D{d}" for d in defines],
+                   "-c", path, "-o", obj]
         else:  # host-only C++ (file formats, Fortran-ABI shim)
             cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC",
                    "-Wall", "-I", os.path.join(ROOT, "include"), "-c", path,
@@ -75,13 +80,13 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
             failed.append(cmd[-3])
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     link = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

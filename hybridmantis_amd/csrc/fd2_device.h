@@ -183,9 +183,10 @@ __device__ __forceinline__ void renorm(float& x, float& y, float& z,
                                        const GeomDev& g) {
   float nrm = sqrtf(x * x + y * y + z * z);
   if (nrm < g.norm_lo || nrm > g.norm_hi) {
-    x = x / nrm;
-    y = y / nrm;
-    z = z / nrm;
+    const float inv = 1.0f / nrm;
+    x = x * inv;
+    y = y * inv;
+    z = z * inv;
   }
 }
 
@@ -224,15 +225,15 @@ __device__ __forceinline__ void rough(Rng& rng, float& nx, float& ny, float& nz,
   float sn, cs;
   fd2_sincos(th, &sn, &cs);
   float qx = rr * cs, qy = rr * sn;
-  float nb = sqrtf(qx * qx + qy * qy + qz * qz);
-  qx = qx / nb;
-  qy = qy / nb;
-  qz = qz / nb;
+  float inv = 1.0f / sqrtf(qx * qx + qy * qy + qz * qz);
+  qx = qx * inv;
+  qy = qy * inv;
+  qz = qz * inv;
   float rx = beta * qx + nx, ry = beta * qy + ny, rz = beta * qz + nz;
-  nb = sqrtf(rx * rx + ry * ry + rz * rz);
-  nx = rx / nb;
-  ny = ry / nb;
-  nz = rz / nb;
+  inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
+  nx = rx * inv;
+  ny = ry * inv;
+  nz = rz * inv;
 }
 
 // isotropic direction at photon termination / trapped restart
@@ -304,8 +305,9 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
   const float qb = 2.0f * (dx0 * p.ux + dy0 * p.uy);
   const float qc = dx0 * dx0 + dy0 * dy0 - R * R;
   const float sq = sqrtf(qb * qb - 4.0f * qa * qc);
-  const float r1 = (-qb + sq) / (2.0f * qa);
-  const float r2 = (-qb - sq) / (2.0f * qa);
+  const float inv2a = 1.0f / (2.0f * qa);
+  const float r1 = (-qb + sq) * inv2a;
+  const float r2 = (-qb - sq) * inv2a;
   float dcyl = 999.0f;
   if (r1 >= r2)
     dcyl = r1;
@@ -454,10 +456,10 @@ __device__ __forceinline__ bool column_normal(Rng& rng, Photon& p,
   for (;;) {
     float nx = mx, ny = my, nz = mz;
     rough(rng, nx, ny, nz, 1.0f);
-    const float den = sqrtf(nx * nx + ny * ny);
+    const float inv = 1.0f / sqrtf(nx * nx + ny * ny);
     nz = 0.0f;
-    nx = nx / den;
-    ny = ny / den;
+    nx = nx * inv;
+    ny = ny * inv;
     if (always_rough || g.beta > 0.0f) rough(rng, nx, ny, nz, g.beta);
     p.nx = nx;
     p.ny = ny;
@@ -643,16 +645,22 @@ __device__ __forceinline__ void fresnel_pq(float t1, float t2, float& pr,
   pt = 0.5f * (s12 / (sp2 * (cm * cm)) + s12 / sp2);
 }
 
-// boundary_analysis (main/kernel_cuda_c_ver1_0.cu:1582-1958).  true = ended.
+// boundary_analysis (main/kernel_cuda_c_ver1_0.cu:1582-1958), split in two.
+// ba_entry is the frequent part -- the top reflector and the crosstalk
+// decision with its new-column normal (K:1610-1673); it returns kBaAlive,
+// kBaEnded or kBaPrpt when the wall's Fresnel branch must follow (K:1677-1955,
+// ba_prpt below).  Running ba_prpt later for a lane changes nothing for
+// that photon: all its state and its random stream go with it.
+enum BaResult : int { kBaAlive = 0, kBaEnded = 1, kBaPrpt = 2 };
+
 template <class Rng>
-__device__ __forceinline__ bool boundary_analysis(Rng& rng, Photon& p,
-                                               const GeomDev& g) {
+__device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
   const float H2 = g.half_h, R = g.R;
   // top reflector reached inside a column (K:1610-1634)
   if (fabsf(p.pz - H2) < g.eps_lt && p.uz > 0.0f) {
     if (g.top_absfrac > 0.0f && rng.u() < g.top_absfrac) {
       p.fate = kAbsTop;
-      return true;
+      return kBaEnded;
     }
     p.nx = 0.0f;
     p.ny = 0.0f;
@@ -664,7 +672,7 @@ __device__ __forceinline__ bool boundary_analysis(Rng& rng, Photon& p,
     fd2_sincos(th, &sn, &cs);
     p.ux = rr * cs;
     p.uy = rr * sn;
-    return false;
+    return kBaAlive;
   }
   // crosstalk probability, piecewise linear in depth (K:1639-1652)
   float cct = 0.0f;
@@ -677,12 +685,19 @@ __device__ __forceinline__ bool boundary_analysis(Rng& rng, Photon& p,
   if (rng.u() < cct) {
     // neighbour column with a random orientation, direction kept (K:2452-2488)
     int tries = 0;
-    if (column_normal(rng, p, g, -p.ux, -p.uy, -p.uz, tries, false)) return true;
+    if (column_normal(rng, p, g, -p.ux, -p.uy, -p.uz, tries, false)) return kBaEnded;
     p.xc = p.px + R * (-p.nx);
     p.yc = p.py + R * (-p.ny);
-    return false;
+    return kBaAlive;
   }
+  return kBaPrpt;
+}
 
+// Fresnel reflection/refraction at a column wall and the flights between
+// columns it leads to (K:1677-1955).  true = ended.
+template <class Rng>
+__device__ __forceinline__ bool ba_prpt(Rng& rng, Photon& p, const GeomDev& g) {
+  const float R = g.R;
   bool inside = true;  // flag_call_transmit
   int theta_tries = 0, smooth_tries = 0, reperturb = 0, normal_tries = 0;
   float pr = 0.0f, pt = 0.0f, t1 = 0.0f, t2 = 0.0f;
@@ -696,9 +711,9 @@ prpt:
   }
   if (inside) {
     const float ax = p.xc - p.px, ay = p.yc - p.py;
-    const float mag = sqrtf(ax * ax + ay * ay);
-    p.nx = ax / mag;
-    p.ny = ay / mag;
+    const float inv = 1.0f / sqrtf(ax * ax + ay * ay);
+    p.nx = ax * inv;
+    p.ny = ay * inv;
     p.nz = 0.0f;
     if (g.beta > 0.0f) rough(rng, p.nx, p.ny, p.nz, g.beta);
   } else {
@@ -797,12 +812,13 @@ no_perturbation:
 
 // One algo() pass of the reference (main/kernel_cuda_c_ver1_0.cu:553-655)
 // fused with the boundary interaction that follows it: in-column step and
-// sensor check, then (if alive and not trapped) boundary_analysis.
-enum StepResult : int { kAlive = 0, kEnded = 1, kTrapped = 2 };
+// sensor check, then (if alive and not trapped) boundary_analysis up to its
+// Fresnel branch; kNeedPrpt asks the caller to run ba_prpt.
+enum StepResult : int { kAlive = 0, kEnded = 1, kTrapped = 2, kNeedPrpt = 3 };
 
 template <class Rng, bool TRACE>
-__device__ __forceinline__ int photon_step(Rng& rng, Photon& p,
-                                           const GeomDev& g, StepSink* sink) {
+__device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
+                                        StepSink* sink) {
   if (++p.loops > kMaxLoops) {
     p.fate = kLost;
     return kEnded;
@@ -811,8 +827,18 @@ __device__ __forceinline__ int photon_step(Rng& rng, Photon& p,
   if (detection<Rng, TRACE>(rng, p, g, sink)) return kEnded;
   if (fabsf(p.uz) < g.eps_lt) return kTrapped;
   p.first = false;
-  if (boundary_analysis(rng, p, g)) return kEnded;
-  return kAlive;
+  const int b = ba_entry(rng, p, g);
+  if (b == kBaEnded) return kEnded;
+  return b == kBaPrpt ? kNeedPrpt : kAlive;
+}
+
+// the whole pass, for kernels that do not defer the Fresnel branch
+template <class Rng, bool TRACE>
+__device__ __forceinline__ int photon_step(Rng& rng, Photon& p,
+                                           const GeomDev& g, StepSink* sink) {
+  const int r = hot_step<Rng, TRACE>(rng, p, g, sink);
+  if (r != kNeedPrpt) return r;
+  return ba_prpt(rng, p, g) ? kEnded : kAlive;
 }
 
 // trapped photon (|dcos.z| < eps): restart from the event with a new

@@ -221,6 +221,7 @@ int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
   a.event_base = (uint32_t)c->next_event;
   a.n_geoms = c->n_geoms;
   a.chunk_log2 = q.chunk_log2;
+  a.park_lanes = c->tune.park_lanes;
 
   const bool multi = c->n_geoms > 1;
   const int per_cu = c->tune.waves_per_cu / (kTransportThreads / 64);
@@ -281,6 +282,7 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
   c->tune.chunk = 64;
   c->tune.waves_per_cu = 16;
   c->tune.lds_tile = 0;
+  c->tune.park_lanes = 32;
   int rc = FD2_OK;
   do {
     if (hipSetDevice(device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
@@ -369,6 +371,10 @@ int fd2_set_tuning(fd2_ctx* c, const fd2_tuning* t) {
   }
   if (t->lds_tile < 0) return FD2_E_INVAL;
   n.lds_tile = t->lds_tile;
+  if (t->park_lanes) {
+    if (t->park_lanes < 1 || t->park_lanes > 64) return FD2_E_INVAL;
+    n.park_lanes = t->park_lanes;
+  }
   c->tune = n;
   return FD2_OK;
 }

@@ -23,6 +23,7 @@ struct TransportArgs {
   uint32_t event_base;                 // global number of events[0]
   int32_t n_geoms;
   int32_t chunk_log2;
+  int32_t park_lanes;                  // lanes waiting before a Fresnel phase
 };
 
 struct QueueArgs {

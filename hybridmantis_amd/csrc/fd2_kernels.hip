@@ -210,8 +210,22 @@ __device__ __forceinline__ void counter_initial_dir(CounterRng& rng, Photon& p,
 
 // ------------------------------------------------------- transport
 
+// Register budget of the transport kernel: 4 waves per SIMD (128 VGPRs) runs
+// faster than 3 waves with no spills, although a few values then live in
+// scratch (measured on MI355X at 1e5 histories).  FD2_WAVES_PER_EU=0 lets
+// the compiler choose.
+#ifndef FD2_WAVES_PER_EU
+#define FD2_WAVES_PER_EU 4
+#endif
+#if FD2_WAVES_PER_EU > 0
+#define FD2_TRANSPORT_ATTR __attribute__((amdgpu_waves_per_eu(FD2_WAVES_PER_EU)))
+#else
+#define FD2_TRANSPORT_ATTR
+#endif
+
 template <bool MULTI, bool REC>
-__global__ void __launch_bounds__(kTransportThreads) k_transport(TransportArgs a) {
+__global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
+k_transport(TransportArgs a) {
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x) s_cnt[i] = 0;
   __syncthreads();
@@ -228,6 +242,7 @@ __global__ void __launch_bounds__(kTransportThreads) k_transport(TransportArgs a
   Photon p;
   CounterRng rng;
   bool alive = false;
+  bool parked = false;  // waiting for the wave's next Fresnel/gap phase
   uint32_t gi = 0;
   unsigned long long rec_idx = 0;
   StepSink nosink{nullptr, nullptr, 0, 0, false};
@@ -275,24 +290,41 @@ __global__ void __launch_bounds__(kTransportThreads) k_transport(TransportArgs a
       need = __ballot(!alive);
     }
     if (__ballot(alive) == 0ull) break;
-    if (alive) {
+    bool ended = false;
+    if (alive && !parked) {
       const GeomDev& g = a.geoms[MULTI ? gi : 0u];
-      const int r = photon_step<CounterRng, false>(rng, p, g, &nosink);
+      const int r = hot_step<CounterRng, false>(rng, p, g, &nosink);
       if (r == kEnded) {
-        atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
-        if (REC) {
-          fd2_photon rec;
-          rec.fate = p.fate;
-          rec.pix = p.pix;
-          rec.x = p.fx;
-          rec.y = p.fy;
-          a.records[rec_idx] = rec;
-        }
-        alive = false;
+        ended = true;
       } else if (r == kTrapped) {
         atomicAdd(&s_cnt[gi * kCounters + FD2_C_REGEN], 1u);
         restart_trapped(rng, p, g);
+      } else if (r == kNeedPrpt) {
+        parked = true;
       }
+    }
+    // The Fresnel/gap branch is long and taken by few lanes at a time: run
+    // it once enough lanes wait for it, or when nothing else can run.
+    const unsigned long long parked_mask = __ballot(parked);
+    if (parked_mask != 0ull &&
+        (__popcll(parked_mask) >= a.park_lanes || __ballot(alive && !parked) == 0ull)) {
+      if (parked) {
+        const GeomDev& g = a.geoms[MULTI ? gi : 0u];
+        ended = ba_prpt(rng, p, g);
+        parked = false;
+      }
+    }
+    if (ended) {
+      atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
+      if (REC) {
+        fd2_photon rec;
+        rec.fate = p.fate;
+        rec.pix = p.pix;
+        rec.x = p.fx;
+        rec.y = p.fy;
+        a.records[rec_idx] = rec;
+      }
+      alive = false;
     }
   }
 

@@ -17,7 +17,8 @@ import numpy as np
 from .events import EVENT_DTYPE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfd2.so")
+# FD2_LIBRARY selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("FD2_LIBRARY") or os.path.join(HERE, "libfd2.so")
 
 PHOTON_DTYPE = np.dtype([("fate", "<i4"), ("pix", "<i4"), ("x", "<f4"),
                          ("y", "<f4")])
@@ -47,7 +48,7 @@ class Params(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("waves_per_cu", C.c_int32),
-                ("lds_tile", C.c_int32), ("reserved", C.c_int32)]
+                ("lds_tile", C.c_int32), ("park_lanes", C.c_int32)]
 
 
 # dict keys (events.BASE_PARAMS) -> fd2_params members
@@ -198,14 +199,16 @@ class Transport:
             pass
 
     # -- configuration
-    def set_tuning(self, chunk: int = 0, waves_per_cu: int = 0, lds_tile: int = 0):
-        t = Tuning(chunk, waves_per_cu, lds_tile, 0)
+    def set_tuning(self, chunk: int = 0, waves_per_cu: int = 0, lds_tile: int = 0,
+                   park_lanes: int = 0):
+        t = Tuning(chunk, waves_per_cu, lds_tile, park_lanes)
         _check(lib().fd2_set_tuning(self._ctx, C.byref(t)), "set_tuning")
 
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().fd2_get_tuning(self._ctx, C.byref(t)), "get_tuning")
-        return dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu, lds_tile=t.lds_tile)
+        return dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu, lds_tile=t.lds_tile,
+                    park_lanes=t.park_lanes)
 
     def set_seed(self, seed: int) -> None:
         _check(lib().fd2_set_seed(self._ctx, seed & 0xFFFFFFFF), "set_seed")

@@ -99,7 +99,8 @@ typedef struct fd2_tuning {
   int32_t chunk;            /* photons per work-queue item (power of 2)     */
   int32_t waves_per_cu;     /* resident waves per CU (workgroups of 4 waves) */
   int32_t lds_tile;         /* side of the LDS PRF tile (0 = off)           */
-  int32_t reserved;
+  int32_t park_lanes;       /* lanes of a wave that wait for the Fresnel /
+                             * inter-column phase before it runs (1-64)     */
 } fd2_tuning;
 
 typedef struct fd2_ctx fd2_ctx;

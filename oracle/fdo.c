@@ -182,12 +182,19 @@ static float dot3(const float *a, const float *b) {  /* K:3110-3117 */
 }
 
 /* renormalise when |v| leaves [1-eps, 1+eps]; the comparison is in double */
-static void renorm(float *v) {
+static void renorm(const ctx *c, float *v) {
   float nrm = (float)sqrt((double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
   if ((double)nrm < 1.0 - FDO_EPS || (double)nrm > 1.0 + FDO_EPS) {
-    v[0] = v[0] / nrm;
-    v[1] = v[1] / nrm;
-    v[2] = v[2] / nrm;
+    if (c->dev) { /* kernel: one reciprocal */
+      const float inv = 1.0f / nrm;
+      v[0] = v[0] * inv;
+      v[1] = v[1] * inv;
+      v[2] = v[2] * inv;
+    } else {
+      v[0] = v[0] / nrm;
+      v[1] = v[1] / nrm;
+      v[2] = v[2] / nrm;
+    }
   }
 }
 
@@ -220,7 +227,7 @@ static void redirect(ctx *c, float t1, float t2, int refract) {
       u[2] = (float)((double)(-n[2]) * ct - ratio * (double)(u[2] + ca * n[2]));
     }
   }
-  renorm(u);
+  renorm(c, u);
 }
 
 /* K:3261-3300 RoughSurface: perturbation p from two draws (z = 2s-1 but the
@@ -246,19 +253,30 @@ static void rough(ctx *c, float *n, float beta) {
     p1 = (float)((double)rr * sin((double)th));
     nb = (float)sqrt(pow(p0, 2) + pow(p1, 2) + pow(p2, 2));
   }
-  p0 = p0 / nb;
-  p1 = p1 / nb;
-  p2 = p2 / nb;
+  if (c->dev) {
+    const float inv = 1.0f / nb;
+    p0 = p0 * inv;
+    p1 = p1 * inv;
+    p2 = p2 * inv;
+  } else {
+    p0 = p0 / nb;
+    p1 = p1 / nb;
+    p2 = p2 / nb;
+  }
   float r0 = beta * p0 + n[0];
   float r1 = beta * p1 + n[1];
   float r2 = beta * p2 + n[2];
-  if (c->dev)
-    nb = sqrtf(r0 * r0 + r1 * r1 + r2 * r2);
-  else
+  if (c->dev) {
+    const float inv = 1.0f / sqrtf(r0 * r0 + r1 * r1 + r2 * r2);
+    n[0] = r0 * inv;
+    n[1] = r1 * inv;
+    n[2] = r2 * inv;
+  } else {
     nb = (float)sqrt(pow(r0, 2) + pow(r1, 2) + pow(r2, 2));
-  n[0] = r0 / nb;
-  n[1] = r1 / nb;
-  n[2] = r2 / nb;
+    n[0] = r0 / nb;
+    n[1] = r1 / nb;
+    n[2] = r2 / nb;
+  }
 }
 
 /* x,y direction cosines rr*(cos th, sin th) with th = u*2pi: double in the
@@ -287,7 +305,7 @@ static void emit_dir(ctx *c) {
   u[2] = r;
   float rr = c->dev ? sqrtf(1.0f - r * r) : (float)sqrt(1.0 - (double)(r * r));
   azimuth(c, rr, uni(c), &u[0], &u[1]);
-  renorm(u);
+  renorm(c, u);
 }
 
 /* first photon of an event, drv:257-277: |2u-1| > 0.01 by rejection and the
@@ -300,7 +318,7 @@ static void emit_first_dir(ctx *c) {
   u[2] = r;
   float rr = (float)sqrt((double)(1.0f - r * r));
   azimuth(c, rr, uni(c), &u[0], &u[1]);
-  renorm(u);
+  renorm(c, u);
 }
 
 /* ------------------------------------------------------ dist_to_surface */
@@ -333,8 +351,9 @@ static float dist_to_surface(ctx *c) {
   float r1, r2;
   if (c->dev) {
     float sqf = sqrtf((qb * qb) - (4.0f * qa * qc));
-    r1 = (-qb + sqf) / (2.0f * qa);
-    r2 = (-qb - sqf) / (2.0f * qa);
+    const float inv2a = 1.0f / (2.0f * qa);
+    r1 = (-qb + sqf) * inv2a;
+    r2 = (-qb - sqf) * inv2a;
   } else {
     double sq = sqrt((double)((qb * qb) - (4.0f * qa * qc)));
     r1 = (float)(((double)(-qb) + sq) / (double)(2.0f * qa));
@@ -480,8 +499,14 @@ static int column_normal(ctx *c, const float *md, int *tries, int always_rough) 
     rough(c, n, 1.0f);
     float den = (float)sqrt((double)(n[0] * n[0] + n[1] * n[1]));
     n[2] = 0.0f;
-    n[0] = n[0] / den;
-    n[1] = n[1] / den;
+    if (c->dev) {
+      const float inv = 1.0f / den;
+      n[0] = n[0] * inv;
+      n[1] = n[1] * inv;
+    } else {
+      n[0] = n[0] / den;
+      n[1] = n[1] / den;
+    }
     if (always_rough || g->beta > 0.0f) rough(c, n, g->beta);
     float ang = c->dev ? fd2_acos(dot3(md, n)) : acosf(dot3(md, n));
     if (ang < 0.0f || ang > 1.57f) {
@@ -766,8 +791,14 @@ prpt:  /* K:1679 */
     c->t->work[FDO_W_WALLNORM]++;
     float mag = (float)sqrt((double)(((c->xc) - p[0]) * ((c->xc) - p[0]) +
                                      ((c->yc) - p[1]) * ((c->yc) - p[1])));
-    n[0] = ((c->xc) - p[0]) / mag;
-    n[1] = ((c->yc) - p[1]) / mag;
+    if (c->dev) {
+      const float inv = 1.0f / mag;
+      n[0] = ((c->xc) - p[0]) * inv;
+      n[1] = ((c->yc) - p[1]) * inv;
+    } else {
+      n[0] = ((c->xc) - p[0]) / mag;
+      n[1] = ((c->yc) - p[1]) / mag;
+    }
     n[2] = 0.0f;
     if (g->beta > 0.0f) rough(c, n, g->beta);
   } else {
