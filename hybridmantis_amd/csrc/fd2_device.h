@@ -150,6 +150,7 @@ struct Photon {
   float nx, ny, nz;   // surface normal
   float xc, yc;       // current column axis
   float ex, ey, ez;   // event position
+  float ax0;          // x of the first column's axis (event x - col_dx)
   int32_t hist;       // 1-based history
   bool first;         // no boundary interaction yet (num_rebound == 0)
   int32_t loops;      // loop passes so far (kMaxLoops cap)
@@ -296,7 +297,7 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
   const float H2 = g.half_h, R = g.R;
   const float t0 = p.px, t1 = p.py, t2 = p.pz;
   if (p.first) {
-    p.xc = p.ex;
+    p.xc = p.ax0;
     p.yc = p.ey;
   }
   const float xc = p.xc, yc = p.yc;

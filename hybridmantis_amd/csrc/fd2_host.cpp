@@ -14,40 +14,6 @@
 
 using namespace fd2;
 
-struct fd2_ctx {
-  int device = 0;
-  int n_cu = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  bool timed = false;
-
-  int n_geoms = 0;
-  fd2_params params[FD2_MAX_GEOMS];
-  GeomDev hgeom[FD2_MAX_GEOMS];
-  GeomDev* d_geoms = nullptr;
-  unsigned long long* d_prf[FD2_MAX_GEOMS] = {};
-  uint32_t* d_hist[FD2_MAX_GEOMS] = {};
-  unsigned long long* d_counters = nullptr;
-
-  // staging and queue buffers, grown on demand
-  fd2_event* h_stage = nullptr;   // pinned
-  size_t cap_stage = 0;
-  fd2_event* d_events = nullptr;
-  size_t cap_events = 0;
-  uint32_t* d_chunk_off = nullptr;
-  unsigned long long* d_photon_off = nullptr;
-  size_t cap_off = 0;
-  uint32_t* d_block_chunks = nullptr;
-  unsigned long long* d_block_photons = nullptr;
-  size_t cap_blocks = 0;
-  uint32_t* d_cursor = nullptr;
-  uint32_t* d_error = nullptr;
-
-  uint32_t seed = 271828182u;
-  int64_t next_event = 0;
-  fd2_tuning tune{};
-};
-
 namespace {
 
 #define FD2_HIP(call)                      \
@@ -191,7 +157,7 @@ int validate_host(const fd2_ctx* c, const fd2_event* ev, int64_t n) {
 
 // queue build + transport on events already in device memory
 int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
-            fd2_photon* d_records) {
+            fd2_photon* d_records, uint32_t* d_radial = nullptr) {
   if (n <= 0) return FD2_OK;
   int rc = grow(c, (size_t)n);
   if (rc) return rc;
@@ -216,6 +182,7 @@ int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
   a.counters = c->d_counters;
   a.cursor = c->d_cursor;
   a.records = d_records;
+  a.radial = d_radial;
   a.n_events = n;
   a.seed = c->seed;
   a.event_base = (uint32_t)c->next_event;
@@ -225,12 +192,13 @@ int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
 
   const bool multi = c->n_geoms > 1;
   const int per_cu = c->tune.waves_per_cu / (kTransportThreads / 64);
-  const int occ = transport_max_blocks_per_cu(multi, record);
+  const bool radial = d_radial != nullptr;
+  const int occ = transport_max_blocks_per_cu(multi, record, radial);
   const int blocks = c->n_cu * (per_cu < 1 ? 1 : (per_cu > occ ? occ : per_cu));
 
   FD2_HIP(hipEventRecord(c->ev_start, c->stream));
   FD2_HIP(launch_queue_build(q, c->stream));
-  FD2_HIP(launch_transport(a, multi, record, blocks, c->stream));
+  FD2_HIP(launch_transport(a, multi, record, radial, blocks, c->stream));
   FD2_HIP(hipEventRecord(c->ev_stop, c->stream));
   c->timed = true;
   c->next_event += n;
@@ -453,6 +421,33 @@ int fd2_submit_record(fd2_ctx* c, const fd2_event* events, int64_t n,
                         hipMemcpyDeviceToHost, c->stream) == hipSuccess ? FD2_OK : FD2_E_DEVICE;
   if (hipStreamSynchronize(c->stream) != hipSuccess) rc = FD2_E_DEVICE;
   if (d_rec) (void)hipFree(d_rec);
+  if (rc) return rc;
+  return check_error_flag(c);
+}
+
+int fd2_submit_radial(fd2_ctx* c, const fd2_event* events, int64_t n,
+                      uint32_t* radial) {
+  if (!c || n < 0 || (n > 0 && (!events || !radial))) return FD2_E_INVAL;
+  if (n == 0) return FD2_OK;
+  int rc = validate_host(c, events, n);
+  if (rc) return rc;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  rc = grow(c, (size_t)n);
+  if (rc) return rc;
+  const size_t nbins = (size_t)n * FD2_RADIAL_BINS;
+  uint32_t* d_rad = nullptr;
+  FD2_HIP(hipMalloc(&d_rad, nbins * sizeof(uint32_t)));
+  if (hipMemcpyAsync(c->d_events, events, (size_t)n * sizeof(fd2_event),
+                     hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      launch_fill_u32(d_rad, nbins, c->stream) != hipSuccess)
+    rc = FD2_E_DEVICE;
+  if (!rc) rc = enqueue(c, c->d_events, n, false, nullptr, d_rad);
+  if (!rc && hipMemcpyAsync(radial, d_rad, nbins * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            c->stream) != hipSuccess)
+    rc = FD2_E_DEVICE;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) rc = FD2_E_DEVICE;
+  (void)hipFree(d_rad);
   if (rc) return rc;
   return check_error_flag(c);
 }

@@ -18,6 +18,7 @@ struct TransportArgs {
   unsigned long long* counters;        // n_geoms * FD2_NCOUNT
   uint32_t* cursor;                    // work-queue cursor (zeroed by the scan)
   fd2_photon* records;                 // per-photon records (recording only)
+  uint32_t* radial;                    // n_events x FD2_RADIAL_BINS (radial mode)
   int64_t n_events;
   uint32_t seed;
   uint32_t event_base;                 // global number of events[0]
@@ -48,7 +49,7 @@ constexpr int kTransportThreads = 256;
 // launch wrappers (fd2_kernels.hip)
 hipError_t launch_queue_build(const QueueArgs& q, hipStream_t s);
 hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
-                            int n_blocks, hipStream_t s);
+                            bool radial, int n_blocks, hipStream_t s);
 hipError_t launch_trace_ranecu(const fd2_event* events, int64_t n,
                                int64_t event_base, int32_t seed_input,
                                const GeomDev* geoms, int32_t n_geoms,
@@ -62,6 +63,41 @@ hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
                               hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
-int transport_max_blocks_per_cu(bool multi, bool record);
+int transport_max_blocks_per_cu(bool multi, bool record, bool radial);
 
 }  // namespace fd2
+
+// The transport context (host side).
+struct fd2_ctx {
+  int device = 0;
+  int n_cu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  bool timed = false;
+
+  int n_geoms = 0;
+  fd2_params params[FD2_MAX_GEOMS];
+  fd2::GeomDev hgeom[FD2_MAX_GEOMS];
+  fd2::GeomDev* d_geoms = nullptr;
+  unsigned long long* d_prf[FD2_MAX_GEOMS] = {};
+  uint32_t* d_hist[FD2_MAX_GEOMS] = {};
+  unsigned long long* d_counters = nullptr;
+
+  // staging and queue buffers, grown on demand
+  fd2_event* h_stage = nullptr;   // pinned
+  size_t cap_stage = 0;
+  fd2_event* d_events = nullptr;
+  size_t cap_events = 0;
+  uint32_t* d_chunk_off = nullptr;
+  unsigned long long* d_photon_off = nullptr;
+  size_t cap_off = 0;
+  uint32_t* d_block_chunks = nullptr;
+  unsigned long long* d_block_photons = nullptr;
+  size_t cap_blocks = 0;
+  uint32_t* d_cursor = nullptr;
+  uint32_t* d_error = nullptr;
+
+  uint32_t seed = 271828182u;
+  int64_t next_event = 0;
+  fd2_tuning tune{};
+};

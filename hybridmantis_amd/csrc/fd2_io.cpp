@@ -182,7 +182,7 @@ int fd2_read_ede(const char* path, fd2_event* events, int64_t max, int64_t* n) {
     e.hist = (int32_t)h;
     e.N = 0;
     e.geom = 0;
-    e.flags = 0;
+    e.col_dx = 0.0f;
   }
   std::fclose(f);
   *n = k;

@@ -175,6 +175,7 @@ __device__ __forceinline__ void start_photon(Photon& p, const fd2_event& ev) {
   p.ex = ev.x;
   p.ey = ev.y;
   p.ez = ev.z;
+  p.ax0 = ev.x - ev.col_dx;
   p.px = ev.x;
   p.py = ev.y;
   p.pz = ev.z;
@@ -223,7 +224,17 @@ __device__ __forceinline__ void counter_initial_dir(CounterRng& rng, Photon& p,
 #define FD2_TRANSPORT_ATTR
 #endif
 
-template <bool MULTI, bool REC>
+// log-radial bin of a detection offset (FD2_RADIAL_BINS layout)
+__device__ __forceinline__ int radial_bin(float dx, float dy) {
+  const float r = sqrtf(dx * dx + dy * dy);
+  if (!(r >= 0.5f)) return 0;
+  if (r >= 200.0f) return FD2_RADIAL_BINS - 1;
+  // 1 + floor(62 * ln(r/0.5) / ln(400))
+  int b = 1 + (int)(fd2_log(r * 2.0f) * 10.3479490f);
+  return b < 1 ? 1 : (b > FD2_RADIAL_BINS - 2 ? FD2_RADIAL_BINS - 2 : b);
+}
+
+template <bool MULTI, bool REC, bool RADIAL>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
 k_transport(TransportArgs a) {
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
@@ -245,6 +256,7 @@ k_transport(TransportArgs a) {
   bool parked = false;  // waiting for the wave's next Fresnel/gap phase
   uint32_t gi = 0;
   unsigned long long rec_idx = 0;
+  int64_t lane_ev = 0;
   StepSink nosink{nullptr, nullptr, 0, 0, false};
 
   for (;;) {
@@ -284,6 +296,7 @@ k_transport(TransportArgs a) {
         rng.seed(a.seed, a.event_base + (uint32_t)pool_ev, k);
         counter_initial_dir(rng, p, k, g);
         if (REC) rec_idx = a.photon_off[pool_ev] + k;
+        if (RADIAL) lane_ev = pool_ev;
         alive = true;
       }
       pool_next += min(__popcll(need), avail);
@@ -316,6 +329,8 @@ k_transport(TransportArgs a) {
     }
     if (ended) {
       atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
+      if (RADIAL && (p.fate == kDetect || p.fate == kDetectOut))
+        atomicAdd(&a.radial[lane_ev * FD2_RADIAL_BINS + radial_bin(p.fx - p.ex, p.fy - p.ey)], 1u);
       if (REC) {
         fd2_photon rec;
         rec.fate = p.fate;
@@ -333,30 +348,40 @@ k_transport(TransportArgs a) {
     if (s_cnt[i]) atomicAdd(&a.counters[i], (unsigned long long)s_cnt[i]);
 }
 
+template <bool MULTI, bool REC, bool RADIAL>
+static void launch_one(const TransportArgs& a, dim3 grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL>), grid, dim3(kTransportThreads), 0, s, a);
+}
+
 hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
-                            int n_blocks, hipStream_t s) {
-  dim3 grid(n_blocks), block(kTransportThreads);
-  if (multi) {
-    if (record) hipLaunchKernelGGL((k_transport<true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_transport<true, false>), grid, block, 0, s, a);
+                            bool radial, int n_blocks, hipStream_t s) {
+  const dim3 grid(n_blocks);
+  if (radial) {
+    if (multi) launch_one<true, false, true>(a, grid, s);
+    else launch_one<false, false, true>(a, grid, s);
+  } else if (multi) {
+    if (record) launch_one<true, true, false>(a, grid, s);
+    else launch_one<true, false, false>(a, grid, s);
   } else {
-    if (record) hipLaunchKernelGGL((k_transport<false, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_transport<false, false>), grid, block, 0, s, a);
+    if (record) launch_one<false, true, false>(a, grid, s);
+    else launch_one<false, false, false>(a, grid, s);
   }
   return hipGetLastError();
 }
 
-int transport_max_blocks_per_cu(bool multi, bool record) {
+template <bool MULTI, bool REC, bool RADIAL>
+static int occupancy_of() {
   int nb = 0;
-  hipError_t e;
-  if (multi) {
-    e = record ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<true, true>, kTransportThreads, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<true, false>, kTransportThreads, 0);
-  } else {
-    e = record ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<false, true>, kTransportThreads, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<false, false>, kTransportThreads, 0);
-  }
-  return e == hipSuccess && nb > 0 ? nb : 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<MULTI, REC, RADIAL>,
+                                                   kTransportThreads, 0) != hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
+}
+
+int transport_max_blocks_per_cu(bool multi, bool record, bool radial) {
+  if (radial) return multi ? occupancy_of<true, false, true>() : occupancy_of<false, false, true>();
+  if (multi) return record ? occupancy_of<true, true, false>() : occupancy_of<true, false, false>();
+  return record ? occupancy_of<false, true, false>() : occupancy_of<false, false, false>();
 }
 
 // ------------------------------------------------ RANECU trajectory mode

@@ -1,0 +1,990 @@
+// fd2_mlp.hip -- the surrogate network: 5 -> 256 -> 256 -> 256 -> 65 MLP
+// with GELU, BF16 weights/activations on the matrix cores (FP32 accumulate,
+// FP32 master weights), trained with Adam by the kernels below.
+//
+// Layout (row-major everywhere, feature dims padded to multiples of 16):
+//   X0 [B x 32] bf16 (5 features + zeros)
+//   W0 [32 x 256], W1, W2 [256 x 256], W3 [256 x 80] (65 outputs + zeros)
+//   Zl = A(l-1) Wl + bl (FP32, kept for backward), Al = bf16(gelu(Zl))
+//   output logits Z4 [B x 80]: column 0 detection logit, 1..64 radial logits
+// Loss per sample: BCE(sigmoid(z0), y0) + w * CE(softmax(z1..64), q) with
+// y0 the detected fraction, q the radial distribution and w = 1 when the
+// sample detected photons (else 0); the batch loss is the mean.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/fd2.h"
+#include "fd2_internal.h"
+#include "fd2_mlp_internal.h"
+
+namespace fd2mlp {
+
+typedef short short4_ __attribute__((ext_vector_type(4)));
+typedef float float4_ __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU, tanh form: 0.5 x (1 + tanh(c (x + 0.044715 x^3))), c = sqrt(2/pi)
+__device__ __forceinline__ float gelu_f(float x) {
+  const float c = 0.7978845608f;
+  const float t = tanhf(c * (x + 0.044715f * x * x * x));
+  return 0.5f * x * (1.0f + t);
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float c = 0.7978845608f;
+  const float u = c * (x + 0.044715f * x * x * x);
+  const float t = tanhf(u);
+  const float du = c * (1.0f + 3.0f * 0.044715f * x * x);
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
+}
+
+// ------------------------------------------------------------- GEMM
+//
+// C[M x N] = op(A)[M x K] * op(B)[K x N], K a multiple of 16.
+// TA: A is stored transposed ([K x M]); TB: B stored transposed ([N x K]).
+// 256 threads = 4 waves compute a 64 x 64 tile; each wave a 32 x 32 quarter
+// as 2 x 2 blocks of v_mfma_f32_16x16x16bf16_1k.  Operand tiles of 64 x 16
+// go through LDS so every layout is read with coalesced 8-byte loads.
+// Split-K: blockIdx.z takes K range [z*kchunk, (z+1)*kchunk).
+enum Epilogue : int {
+  kStoreF32 = 0,      // C (+bias) -> outf; gelu(C) as bf16 -> outb if set
+  kAtomicF32 = 1,     // outf += C (split-K weight gradients)
+  kGeluGradBf16 = 2,  // outb = bf16(C * gelu'(Z)) and bias grad += colsum
+};
+
+struct GemmArgs {
+  const uint16_t* A;
+  const uint16_t* B;
+  int M, N, K;
+  int lda, ldb;        // leading dims of the stored arrays
+  int kchunk;
+  const float* bias;   // kStoreF32
+  float* outf;         // kStoreF32 / kAtomicF32 (ld = N)
+  uint16_t* outb;      // kStoreF32 (gelu) / kGeluGradBf16 (ld = ldo)
+  const float* Z;      // kGeluGradBf16: pre-activations (ld = N)
+  float* dbias;        // kGeluGradBf16: column sums of the product
+  int ldo;
+};
+
+template <bool TA, bool TB, int EPI>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ uint16_t As[64][16 + 4];
+  __shared__ uint16_t Bs[16][64 + 4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int kbeg = blockIdx.z * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  float4_ acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4_{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = kbeg; k0 < kend; k0 += 16) {
+    // A tile: 64 rows x 16 k
+    if (!TA) {
+      const int r = t >> 2, kk = (t & 3) * 4;
+      const int gm = m0 + r;
+      uint16_t v[4] = {0, 0, 0, 0};
+      if (gm < g.M) {
+        const uint2 q = *reinterpret_cast<const uint2*>(g.A + (size_t)gm * g.lda + k0 + kk);
+        v[0] = q.x & 0xFFFF; v[1] = q.x >> 16; v[2] = q.y & 0xFFFF; v[3] = q.y >> 16;
+      }
+      for (int j = 0; j < 4; ++j) As[r][kk + j] = v[j];
+    } else {
+      const int kk = t >> 4, r = (t & 15) * 4;
+      const int gm = m0 + r;
+      const uint16_t* src = g.A + (size_t)(k0 + kk) * g.lda + gm;
+      for (int j = 0; j < 4; ++j) As[r + j][kk] = (gm + j < g.M) ? src[j] : (uint16_t)0;
+    }
+    // B tile: 16 k x 64 cols
+    if (!TB) {
+      const int kk = t >> 4, c = (t & 15) * 4;
+      const int gn = n0 + c;
+      const uint16_t* src = g.B + (size_t)(k0 + kk) * g.ldb + gn;
+      for (int j = 0; j < 4; ++j) Bs[kk][c + j] = (gn + j < g.N) ? src[j] : (uint16_t)0;
+    } else {
+      const int c = t >> 2, kk = (t & 3) * 4;
+      const int gn = n0 + c;
+      uint16_t v[4] = {0, 0, 0, 0};
+      if (gn < g.N) {
+        const uint2 q = *reinterpret_cast<const uint2*>(g.B + (size_t)gn * g.ldb + k0 + kk);
+        v[0] = q.x & 0xFFFF; v[1] = q.x >> 16; v[2] = q.y & 0xFFFF; v[3] = q.y >> 16;
+      }
+      for (int j = 0; j < 4; ++j) Bs[kk + j][c] = v[j];
+    }
+    __syncthreads();
+    // fragments: A lane l -> row l%16, k 4*(l/16)..+3; B lane l -> col l%16
+    short4_ af[2], bfr[2];
+    const int lr = lane & 15, lk = (lane >> 4) * 4;
+    for (int i = 0; i < 2; ++i) {
+      const uint16_t* ap = &As[wm + i * 16 + lr][lk];
+      af[i] = short4_{(short)ap[0], (short)ap[1], (short)ap[2], (short)ap[3]};
+      bfr[i] = short4_{(short)Bs[lk][wn + i * 16 + lr], (short)Bs[lk + 1][wn + i * 16 + lr],
+                       (short)Bs[lk + 2][wn + i * 16 + lr], (short)Bs[lk + 3][wn + i * 16 + lr]};
+    }
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // D lane l: col l%16, rows 4*(l/16)+r
+  const int lc = lane & 15, lr4 = (lane >> 4) * 4;
+  for (int i = 0; i < 2; ++i) {
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + j * 16 + lc;
+      float colsum = 0.0f;
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lr4 + r;
+        const float v = acc[i][j][r];
+        if (row >= g.M || col >= g.N) continue;
+        if (EPI == kStoreF32) {
+          const float z = v + (g.bias ? g.bias[col] : 0.0f);
+          g.outf[(size_t)row * g.N + col] = z;
+          if (g.outb) g.outb[(size_t)row * g.ldo + col] = f2bf(gelu_f(z));
+        } else if (EPI == kAtomicF32) {
+          atomicAdd(&g.outf[(size_t)row * g.N + col], v);
+        } else {
+          const float d = v * gelu_grad(g.Z[(size_t)row * g.N + col]);
+          const uint16_t h = f2bf(d);
+          g.outb[(size_t)row * g.ldo + col] = h;
+          colsum += bf2f(h);
+        }
+      }
+      if (EPI == kGeluGradBf16 && g.dbias) {
+        // lanes l, l+16, l+32, l+48 share the column
+        colsum += __shfl_xor(colsum, 16, 64);
+        colsum += __shfl_xor(colsum, 32, 64);
+        if ((lane >> 4) == 0 && col < g.N) atomicAdd(&g.dbias[col], colsum);
+      }
+    }
+  }
+}
+
+template <bool TA, bool TB, int EPI>
+static hipError_t gemm(const GemmArgs& a, int splits, hipStream_t s) {
+  dim3 grid((a.N + 63) / 64, (a.M + 63) / 64, splits);
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------- elementwise kernels
+
+// features (FP32 [n x 5], host-normalised) -> X0 bf16 [n x 32]
+__global__ void k_pack_inputs(const float* x, int n, uint16_t* x0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * kIn) return;
+  const int row = i / kIn, c = i % kIn;
+  x0[(size_t)row * kInPad + c] = c < kFeat ? f2bf(x[(size_t)row * kFeat + c]) : (uint16_t)0;
+}
+
+// gather a minibatch: rows idx[b] of the resident data set
+__global__ void k_gather(const float* xs, const float* ys, const uint32_t* idx, int B,
+                         uint16_t* x0, float* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B) return;
+  const uint32_t r = idx[i];
+  for (int c = 0; c < kInPad; ++c)
+    x0[(size_t)i * kInPad + c] = c < kFeat ? f2bf(xs[(size_t)r * kFeat + c]) : (uint16_t)0;
+  for (int c = 0; c < kTarget; ++c) y[(size_t)i * kTarget + c] = ys[(size_t)r * kTarget + c];
+}
+
+// loss and dL/dZ4 for one sample per wave: lane k handles radial logit k
+// (z[1+k]); lane 0 also the detection logit.  dz -> bf16 [B x 80];
+// column 0 and 1..64 bias gradients accumulated.
+__global__ void __launch_bounds__(256) k_head_loss(const float* z4, const float* y, int B,
+                                                   uint16_t* dz, float* db3, float* loss) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= B) return;
+  const float* zr = z4 + (size_t)wave * kOutPad;
+  const float* yr = y + (size_t)wave * kTarget;
+  const float invB = 1.0f / (float)B;
+  // radial softmax over 64 lanes
+  const float zl = zr[1 + lane];
+  float mx = zl;
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  const float e = expf(zl - mx);
+  float se = e;
+  for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
+  const float prob = e / se;
+  const float q = yr[1 + lane];
+  float qs = q;
+  for (int o = 32; o > 0; o >>= 1) qs += __shfl_xor(qs, o, 64);
+  const float wrad = qs > 0.0f ? 1.0f : 0.0f;
+  float ce = q > 0.0f ? -q * (zl - mx - logf(se)) : 0.0f;
+  for (int o = 32; o > 0; o >>= 1) ce += __shfl_xor(ce, o, 64);
+  const float gr = wrad * (prob - q) * invB;
+  uint16_t* dr = dz + (size_t)wave * kOutPad;
+  dr[1 + lane] = f2bf(gr);
+  atomicAdd(&db3[1 + lane], bf2f(f2bf(gr)));
+  if (lane < kOutPad - 1 - 64) dr[65 + lane] = 0;
+  if (lane == 0) {
+    const float z0 = zr[0], y0 = yr[0];
+    const float p = 1.0f / (1.0f + expf(-z0));
+    // BCE with logits: max(z,0) - z y + log(1 + e^-|z|)
+    const float bce = fmaxf(z0, 0.0f) - z0 * y0 + log1pf(expf(-fabsf(z0)));
+    const float g0 = (p - y0) * invB;
+    dr[0] = f2bf(g0);
+    atomicAdd(&db3[0], bf2f(f2bf(g0)));
+    atomicAdd(loss, (bce + wrad * ce) * invB);
+  }
+}
+
+// logits -> [p_det, radial probabilities] (FP32 [n x 65])
+__global__ void __launch_bounds__(256) k_head_probs(const float* z4, int n, float* out) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= n) return;
+  const float* zr = z4 + (size_t)wave * kOutPad;
+  const float zl = zr[1 + lane];
+  float mx = zl;
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  const float e = expf(zl - mx);
+  float se = e;
+  for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
+  float* orow = out + (size_t)wave * kTarget;
+  orow[1 + lane] = e / se;
+  if (lane == 0) orow[0] = 1.0f / (1.0f + expf(-zr[0]));
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the bf16 copy
+// of weight matrices and clears the gradient.
+__global__ void k_adam(float* w, float* g, float* m, float* v, uint16_t* wb, int n,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  g[i] = 0.0f;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ void k_to_bf16(const float* w, uint16_t* wb, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) wb[i] = f2bf(w[i]);
+}
+
+__global__ void k_zero(float* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+}
+
+}  // namespace fd2mlp
+
+using namespace fd2mlp;
+
+// -------------------------------------------------------------- host side
+
+struct fd2_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cap = 0;  // batch capacity
+  float* w = nullptr;   // FP32 master parameters (kParams)
+  float* g = nullptr;   // gradients
+  float* m = nullptr;   // Adam moments
+  float* v = nullptr;
+  uint16_t* wb = nullptr;  // bf16 copy (same layout as w)
+  uint16_t* x0 = nullptr;
+  float* z[4] = {};
+  uint16_t* a[3] = {};
+  uint16_t* dz = nullptr;    // bf16 gradient of the current layer's output
+  uint16_t* dz2 = nullptr;
+  float* y = nullptr;
+  float* loss = nullptr;
+  float* data_x = nullptr;
+  float* data_y = nullptr;
+  int64_t n_data = 0;
+  uint32_t* idx = nullptr;
+  int64_t step = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+#define MLP_HIP(call)                          \
+  do {                                         \
+    if ((call) != hipSuccess) return FD2_E_DEVICE; \
+  } while (0)
+
+void free_batch(fd2_mlp* p) {
+  void* bufs[] = {p->x0, p->z[0], p->z[1], p->z[2], p->z[3], p->a[0], p->a[1], p->a[2],
+                  p->dz, p->dz2, p->y, p->idx};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  p->x0 = nullptr;
+  for (auto& q : p->z) q = nullptr;
+  for (auto& q : p->a) q = nullptr;
+  p->dz = p->dz2 = nullptr;
+  p->y = nullptr;
+  p->idx = nullptr;
+  p->cap = 0;
+}
+
+int ensure_batch(fd2_mlp* p, int B) {
+  if (B <= p->cap) return FD2_OK;
+  free_batch(p);
+  const size_t b = (size_t)B;
+  MLP_HIP(hipMalloc(&p->x0, b * kInPad * 2));
+  for (int l = 0; l < 3; ++l) {
+    MLP_HIP(hipMalloc(&p->z[l], b * kHidden * 4));
+    MLP_HIP(hipMalloc(&p->a[l], b * kHidden * 2));
+  }
+  MLP_HIP(hipMalloc(&p->z[3], b * kOutPad * 4));
+  MLP_HIP(hipMalloc(&p->dz, b * kHidden * 2));
+  MLP_HIP(hipMalloc(&p->dz2, b * kHidden * 2));
+  MLP_HIP(hipMalloc(&p->y, b * kTarget * 4));
+  MLP_HIP(hipMalloc(&p->idx, b * 4));
+  p->cap = B;
+  return FD2_OK;
+}
+
+// forward pass on p->x0 (B rows) -> z[3] logits
+int forward(fd2_mlp* p, int B, hipStream_t stream) {
+  const int ins[4] = {kInPad, kHidden, kHidden, kHidden};
+  const int outs[4] = {kHidden, kHidden, kHidden, kOutPad};
+  const uint16_t* in = p->x0;
+  for (int l = 0; l < 4; ++l) {
+    GemmArgs a{};
+    a.A = in;
+    a.B = p->wb + kWOff[l];
+    a.M = B;
+    a.N = outs[l];
+    a.K = ins[l];
+    a.lda = ins[l];
+    a.ldb = outs[l];
+    a.kchunk = ins[l];
+    a.bias = p->w + kBOff[l];
+    a.outf = p->z[l];
+    a.outb = l < 3 ? p->a[l] : nullptr;
+    a.ldo = outs[l];
+    MLP_HIP((gemm<false, false, kStoreF32>(a, 1, stream)));
+    in = l < 3 ? p->a[l] : nullptr;
+  }
+  return FD2_OK;
+}
+
+// backward from the bf16 output gradient in p->dz (B x 80) into p->g
+int backward(fd2_mlp* p, int B) {
+  const int ins[4] = {kInPad, kHidden, kHidden, kHidden};
+  const int outs[4] = {kHidden, kHidden, kHidden, kOutPad};
+  if (B % 16) return FD2_E_INVAL;
+  const int target = B >= 1024 ? 16 : 1;
+  const int kchunk = ((B + target - 1) / target + 15) / 16 * 16;
+  const int splits = (B + kchunk - 1) / kchunk;
+  uint16_t* dcur = p->dz;   // gradient w.r.t. layer l's output Z_l
+  uint16_t* dnext = p->dz2;
+  for (int l = 3; l >= 0; --l) {
+    const uint16_t* in = l == 0 ? p->x0 : p->a[l - 1];
+    // dW_l = in^T dZ_l  ([ins x B] x [B x outs]), split over B
+    GemmArgs gw{};
+    gw.A = in;
+    gw.B = dcur;
+    gw.M = ins[l];
+    gw.N = outs[l];
+    gw.K = B;
+    gw.lda = ins[l];
+    gw.ldb = outs[l];
+    gw.kchunk = kchunk;
+    gw.outf = p->g + kWOff[l];
+    MLP_HIP((gemm<true, false, kAtomicF32>(gw, splits, p->stream)));
+    if (l == 0) break;
+    // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1}); bias grad of layer l-1
+    GemmArgs gx{};
+    gx.A = dcur;
+    gx.B = p->wb + kWOff[l];
+    gx.M = B;
+    gx.N = ins[l];
+    gx.K = outs[l];
+    gx.lda = outs[l];
+    gx.ldb = outs[l];
+    gx.kchunk = outs[l];
+    gx.Z = p->z[l - 1];
+    gx.outb = dnext;
+    gx.ldo = ins[l];
+    gx.dbias = p->g + kBOff[l - 1];
+    MLP_HIP((gemm<false, true, kGeluGradBf16>(gx, 1, p->stream)));
+    uint16_t* t = dcur;
+    dcur = dnext;
+    dnext = t;
+  }
+  return FD2_OK;
+}
+
+int loss_grad(fd2_mlp* p, int B, float* d_loss) {
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, p->stream, d_loss, (size_t)1);
+  hipLaunchKernelGGL(k_head_loss, dim3((B * 64 + 255) / 256), dim3(256), 0, p->stream,
+                     p->z[3], p->y, B, p->dz, p->g + kBOff[3], d_loss);
+  MLP_HIP(hipGetLastError());
+  return FD2_OK;
+}
+
+void init_params(std::vector<float>& w, uint32_t seed) {
+  // Glorot-uniform weights, zero biases (padded rows/cols stay zero)
+  uint64_t s = 0x9E3779B97F4A7C15ull ^ seed;
+  auto rnd = [&]() {
+    s ^= s >> 12; s ^= s << 25; s ^= s >> 27;
+    return (double)((s * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const int ins[4] = {kFeat, kHidden, kHidden, kHidden};
+  const int insp[4] = {kInPad, kHidden, kHidden, kHidden};
+  const int outs[4] = {kHidden, kHidden, kHidden, kOut};
+  const int outsp[4] = {kHidden, kHidden, kHidden, kOutPad};
+  std::fill(w.begin(), w.end(), 0.0f);
+  for (int l = 0; l < 4; ++l) {
+    const double lim = std::sqrt(6.0 / (ins[l] + outs[l]));
+    for (int i = 0; i < ins[l]; ++i)
+      for (int j = 0; j < outs[l]; ++j)
+        w[kWOff[l] + (size_t)i * outsp[l] + j] = (float)((2.0 * rnd() - 1.0) * lim);
+    (void)insp;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t fd2_mlp_num_params(void) { return kParams; }
+
+int fd2_mlp_create(int device, uint32_t seed, fd2_mlp** out) {
+  if (!out) return FD2_E_INVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FD2_E_NO_DEVICE;
+  if (device < 0 || device >= ndev) return FD2_E_INVAL;
+  fd2_mlp* p = new (std::nothrow) fd2_mlp();
+  if (!p) return FD2_E_NOMEM;
+  p->device = device;
+  int rc = FD2_OK;
+  do {
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&p->e0) != hipSuccess || hipEventCreate(&p->e1) != hipSuccess) {
+      rc = FD2_E_DEVICE;
+      break;
+    }
+    const size_t n = kParams;
+    if (hipMalloc(&p->w, n * 4) != hipSuccess || hipMalloc(&p->g, n * 4) != hipSuccess ||
+        hipMalloc(&p->m, n * 4) != hipSuccess || hipMalloc(&p->v, n * 4) != hipSuccess ||
+        hipMalloc(&p->wb, n * 2) != hipSuccess || hipMalloc(&p->loss, 4) != hipSuccess) {
+      rc = FD2_E_NOMEM;
+      break;
+    }
+    std::vector<float> h(n);
+    init_params(h, seed);
+    rc = fd2_mlp_set_params(p, h.data(), n);
+  } while (0);
+  if (rc) {
+    fd2_mlp_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return FD2_OK;
+}
+
+void fd2_mlp_destroy(fd2_mlp* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  free_batch(p);
+  void* bufs[] = {p->w, p->g, p->m, p->v, p->wb, p->loss, p->data_x, p->data_y};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (p->e0) (void)hipEventDestroy(p->e0);
+  if (p->e1) (void)hipEventDestroy(p->e1);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+int fd2_mlp_set_params(fd2_mlp* p, const float* host, size_t n) {
+  if (!p || !host || n != kParams) return FD2_E_INVAL;
+  MLP_HIP(hipSetDevice(p->device));
+  MLP_HIP(hipMemcpyAsync(p->w, host, n * 4, hipMemcpyHostToDevice, p->stream));
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
+                     p->w, p->wb, (int)n);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, p->stream, p->g, n);
+  hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, p->stream, p->m, n);
+  hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, p->stream, p->v, n);
+  MLP_HIP(hipGetLastError());
+  MLP_HIP(hipStreamSynchronize(p->stream));
+  p->step = 0;
+  return FD2_OK;
+}
+
+int fd2_mlp_get_params(fd2_mlp* p, float* host, size_t n) {
+  if (!p || !host || n != kParams) return FD2_E_INVAL;
+  MLP_HIP(hipSetDevice(p->device));
+  MLP_HIP(hipStreamSynchronize(p->stream));
+  MLP_HIP(hipMemcpy(host, p->w, n * 4, hipMemcpyDeviceToHost));
+  return FD2_OK;
+}
+
+int fd2_mlp_forward(fd2_mlp* p, const float* x, int64_t n, float* out) {
+  if (!p || n < 0 || (n > 0 && (!x || !out))) return FD2_E_INVAL;
+  MLP_HIP(hipSetDevice(p->device));
+  const int64_t chunk = 65536;
+  float* dx = nullptr;
+  float* dout = nullptr;
+  int rc = ensure_batch(p, (int)chunk);
+  if (rc) return rc;
+  MLP_HIP(hipMalloc(&dx, (size_t)chunk * kFeat * 4));
+  if (hipMalloc(&dout, (size_t)chunk * kTarget * 4) != hipSuccess) {
+    (void)hipFree(dx);
+    return FD2_E_NOMEM;
+  }
+  for (int64_t s = 0; s < n && rc == FD2_OK; s += chunk) {
+    const int b = (int)(n - s < chunk ? n - s : chunk);
+    if (hipMemcpyAsync(dx, x + s * kFeat, (size_t)b * kFeat * 4, hipMemcpyHostToDevice,
+                       p->stream) != hipSuccess) {
+      rc = FD2_E_DEVICE;
+      break;
+    }
+    hipLaunchKernelGGL(k_pack_inputs, dim3((b * kIn + 255) / 256), dim3(256), 0, p->stream,
+                       dx, b, p->x0);
+    rc = forward(p, b, p->stream);
+    if (rc) break;
+    hipLaunchKernelGGL(k_head_probs, dim3((b * 64 + 255) / 256), dim3(256), 0, p->stream,
+                       p->z[3], b, dout);
+    if (hipMemcpyAsync(out + s * kTarget, dout, (size_t)b * kTarget * 4,
+                       hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
+      rc = FD2_E_DEVICE;
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dout);
+  return rc;
+}
+
+// gradients of the mean batch loss (no update): grads (kParams) and loss
+int fd2_mlp_gradients(fd2_mlp* p, const float* x, const float* y, int B, float* grads,
+                      float* loss) {
+  if (!p || !x || !y || !grads || B <= 0 || B % 16) return FD2_E_INVAL;
+  MLP_HIP(hipSetDevice(p->device));
+  int rc = ensure_batch(p, B);
+  if (rc) return rc;
+  float* dx = nullptr;
+  MLP_HIP(hipMalloc(&dx, (size_t)B * kFeat * 4));
+  rc = FD2_OK;
+  if (hipMemcpyAsync(dx, x, (size_t)B * kFeat * 4, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
+      hipMemcpyAsync(p->y, y, (size_t)B * kTarget * 4, hipMemcpyHostToDevice, p->stream) != hipSuccess)
+    rc = FD2_E_DEVICE;
+  if (!rc) {
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((kParams + 255) / 256)), dim3(256), 0, p->stream,
+                       p->g, (size_t)kParams);
+    hipLaunchKernelGGL(k_pack_inputs, dim3((B * kIn + 255) / 256), dim3(256), 0, p->stream,
+                       dx, B, p->x0);
+    rc = forward(p, B, p->stream);
+  }
+  if (!rc) rc = loss_grad(p, B, p->loss);
+  if (!rc) rc = backward(p, B);
+  if (!rc && (hipMemcpyAsync(grads, p->g, kParams * 4, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+              (loss && hipMemcpyAsync(loss, p->loss, 4, hipMemcpyDeviceToHost, p->stream) != hipSuccess) ||
+              hipStreamSynchronize(p->stream) != hipSuccess))
+    rc = FD2_E_DEVICE;
+  (void)hipFree(dx);
+  return rc;
+}
+
+int fd2_mlp_set_data(fd2_mlp* p, const float* x, const float* y, int64_t n) {
+  if (!p || !x || !y || n <= 0) return FD2_E_INVAL;
+  MLP_HIP(hipSetDevice(p->device));
+  if (p->data_x) (void)hipFree(p->data_x);
+  if (p->data_y) (void)hipFree(p->data_y);
+  p->data_x = p->data_y = nullptr;
+  MLP_HIP(hipMalloc(&p->data_x, (size_t)n * kFeat * 4));
+  MLP_HIP(hipMalloc(&p->data_y, (size_t)n * kTarget * 4));
+  MLP_HIP(hipMemcpy(p->data_x, x, (size_t)n * kFeat * 4, hipMemcpyHostToDevice));
+  MLP_HIP(hipMemcpy(p->data_y, y, (size_t)n * kTarget * 4, hipMemcpyHostToDevice));
+  p->n_data = n;
+  return FD2_OK;
+}
+
+// `steps` Adam steps on minibatches of B rows drawn (host RNG, seeded) from
+// the resident data set; losses[steps] receives each step's batch loss
+// (may be NULL); *ms the GPU time of the steps (may be NULL).
+int fd2_mlp_train(fd2_mlp* p, int steps, int B, float lr, uint32_t seed, float* losses,
+                  float* ms) {
+  if (!p || steps < 0 || B <= 0 || B % 16 || p->n_data <= 0) return FD2_E_INVAL;
+  MLP_HIP(hipSetDevice(p->device));
+  int rc = ensure_batch(p, B);
+  if (rc) return rc;
+  // all minibatch indices drawn up front (xorshift64*, seeded)
+  std::vector<uint32_t> idx((size_t)B * steps);
+  uint64_t s = 0xD1B54A32D192ED03ull ^ ((uint64_t)seed << 17) ^ (uint64_t)p->step;
+  for (auto& v : idx) {
+    s ^= s >> 12; s ^= s << 25; s ^= s >> 27;
+    v = (uint32_t)(((s * 2685821657736338717ull) >> 32) % (uint64_t)p->n_data);
+  }
+  uint32_t* d_idx = nullptr;
+  float* d_losses = nullptr;
+  MLP_HIP(hipMalloc(&d_idx, idx.size() * 4 + 4));
+  if (hipMalloc(&d_losses, (size_t)(steps + 1) * 4) != hipSuccess) {
+    (void)hipFree(d_idx);
+    return FD2_E_NOMEM;
+  }
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  rc = hipMemcpyAsync(d_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, p->stream) ==
+               hipSuccess && hipEventRecord(p->e0, p->stream) == hipSuccess
+           ? FD2_OK
+           : FD2_E_DEVICE;
+  for (int st = 0; st < steps && rc == FD2_OK; ++st) {
+    hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, p->stream, p->data_x,
+                       p->data_y, d_idx + (size_t)st * B, B, p->x0, p->y);
+    rc = forward(p, B, p->stream);
+    if (!rc) rc = loss_grad(p, B, d_losses + st);
+    if (!rc) rc = backward(p, B);
+    if (rc) break;
+    p->step++;
+    const float c1 = 1.0f - std::pow(b1, (float)p->step), c2 = 1.0f - std::pow(b2, (float)p->step);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((kParams + 255) / 256)), dim3(256), 0, p->stream,
+                       p->w, p->g, p->m, p->v, p->wb, (int)kParams, lr, b1, b2, eps, c1, c2);
+    if (hipGetLastError() != hipSuccess) rc = FD2_E_DEVICE;
+  }
+  if (!rc && (hipEventRecord(p->e1, p->stream) != hipSuccess ||
+              (losses && hipMemcpyAsync(losses, d_losses, (size_t)steps * 4,
+                                        hipMemcpyDeviceToHost, p->stream) != hipSuccess) ||
+              hipStreamSynchronize(p->stream) != hipSuccess))
+    rc = FD2_E_DEVICE;
+  if (!rc && ms && hipEventElapsedTime(ms, p->e0, p->e1) != hipSuccess) rc = FD2_E_DEVICE;
+  (void)hipStreamSynchronize(p->stream);
+  (void)hipFree(d_idx);
+  (void)hipFree(d_losses);
+  return rc;
+}
+
+}  // extern "C"
+
+// ===================================================== surrogate transport
+
+namespace fd2mlp {
+
+constexpr int kSampleThreads = 256;
+constexpr int kTileDefault = 96;
+
+// network features of an event (see fd2.h): depth below the top / H,
+// col_dx / R, H / 500 um, R / 5 um, top reflectivity
+__device__ __forceinline__ float feature(const fd2_event& e, const fd2::GeomDev& g, int c) {
+  switch (c) {
+    case 0: return (g.half_h - e.z) / g.H;
+    case 1: return e.col_dx / g.R;
+    case 2: return g.H * (1.0f / 500.0f);
+    case 3: return g.R * (1.0f / 5.0f);
+    case 4: return 1.0f - g.top_absfrac;
+    default: return 0.0f;
+  }
+}
+
+__global__ void k_surr_features(const fd2_event* ev, int n, const fd2::GeomDev* geoms,
+                                uint16_t* x0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * kInPad) return;
+  const int row = i / kInPad, c = i % kInPad;
+  const fd2_event e = ev[row];
+  x0[i] = c < kFeat ? f2bf(feature(e, geoms[e.geom], c)) : (uint16_t)0;
+}
+
+// edges of the radial bins: 0, 0.5 * 400^((j-1)/62) for j = 1..63, 300 um
+__device__ __forceinline__ float radial_edge(int j) {
+  if (j <= 0) return 0.0f;
+  if (j >= FD2_RADIAL_BINS) return 300.0f;
+  return 0.5f * expf((float)(j - 1) * 0.0966378962f);  // ln(400)/62
+}
+
+struct SampleArgs {
+  const fd2_event* events;
+  const uint32_t* ids;       // global event numbers (random-stream keys)
+  const float* logits;       // n x kOutPad
+  const fd2::GeomDev* geoms;
+  unsigned long long* counters;
+  int n;
+  int n_geoms;
+  uint32_t seed;
+  int tile;                  // LDS PRF tile side (geometry 0), 0 = none
+  int tile_i0, tile_j0;      // pixel origin of the tile
+};
+
+// One wave per event: lanes are the 64 radial bins for the softmax and its
+// CDF; the binomial detected count is a sum of N Bernoulli(p_det) draws
+// spread over the lanes; each detected photon gets a radius by inverse CDF
+// (uniform in area within its bin) and a uniform azimuth, and is tallied
+// like a Monte Carlo detection.  A workgroup loops over many events and
+// keeps the central part of geometry 0's PRF in LDS.
+__global__ void __launch_bounds__(kSampleThreads) k_surr_sample(SampleArgs a) {
+  extern __shared__ uint32_t s_tile[];
+  __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * 3];
+  const int T = a.tile, T2 = a.tile * a.tile;
+  for (int i = threadIdx.x; i < T2; i += blockDim.x) s_tile[i] = 0u;
+  for (int i = threadIdx.x; i < a.n_geoms * 3; i += blockDim.x) s_cnt[i] = 0u;
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int waves = gridDim.x * (kSampleThreads / 64);
+  for (int e = blockIdx.x * (kSampleThreads / 64) + w; e < a.n; e += waves) {
+    const fd2_event ev = a.events[e];
+    const fd2::GeomDev& g = a.geoms[ev.geom];
+    const float* z = a.logits + (size_t)e * kOutPad;
+    const float zl = z[1 + lane];
+    float mx = zl;
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    const float ex = expf(zl - mx);
+    float cdf = ex;  // inclusive scan over lanes
+    for (int o = 1; o < 64; o <<= 1) {
+      const float t = __shfl_up(cdf, o, 64);
+      if (lane >= o) cdf += t;
+    }
+    const float total = __shfl(cdf, 63, 64);
+    cdf = lane == 63 ? 1.0f : cdf / total;
+    const float pdet = 1.0f / (1.0f + expf(-z[0]));
+    fd2::CounterRng rng;
+    rng.seed(a.seed ^ 0x53555252u, a.ids[e], (uint32_t)lane);
+    int cnt = 0;
+    for (int j = lane; j < ev.N; j += 64) cnt += rng.u() < pdet ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    const int K = cnt;
+    for (int j0 = 0; j0 < K; j0 += 64) {
+      const bool act = j0 + lane < K;
+      const float u = act ? rng.u() : 2.0f;
+      // first bin whose CDF reaches u (binary search over the lanes' CDF)
+      int lo = 0;
+      for (int step = 32; step > 0; step >>= 1) {
+        const float c = __shfl(cdf, lo + step - 1, 64);
+        if (c < u) lo += step;
+      }
+      if (!act) continue;
+      const int bin = lo > 63 ? 63 : lo;
+      const float r0 = radial_edge(bin), r1 = radial_edge(bin + 1);
+      const float r = sqrtf(r0 * r0 + rng.u() * (r1 * r1 - r0 * r0));
+      float sn, cs;
+      fd2_sincos(rng.u() * 6.283185307f, &sn, &cs);
+      const float x = ev.x + r * cs, y = ev.y + r * sn;
+      const int pi = (int)floorf((x - g.lbx) / (float)g.pixel);
+      const int pj = (int)floorf((y - g.lby) / (float)g.pixel);
+      if (x <= g.ubx && y <= g.uby && x >= g.lbx && y >= g.lby && pi >= 0 && pi < g.xdim &&
+          pj >= 0 && pj < g.ydim) {
+        const int ti = pi - a.tile_i0, tj = pj - a.tile_j0;
+        if (ev.geom == 0 && ti >= 0 && ti < T && tj >= 0 && tj < T)
+          atomicAdd(&s_tile[ti * T + tj], 1u);
+        else
+          atomicAdd(&g.prf[(size_t)pi * g.ydim + pj], 1ull);
+      }
+    }
+    if (lane == 0) {
+      if (ev.hist >= 1 && ev.hist <= g.num_hist && K > 0) atomicAdd(&g.hist[ev.hist - 1], (uint32_t)K);
+      atomicAdd(&s_cnt[ev.geom * 3 + 0], (uint32_t)ev.N);
+      atomicAdd(&s_cnt[ev.geom * 3 + 1], (uint32_t)K);
+      atomicAdd(&s_cnt[ev.geom * 3 + 2], (uint32_t)(ev.N - K));
+    }
+  }
+  __syncthreads();
+  if (T > 0) {
+    const fd2::GeomDev& g0 = a.geoms[0];
+    for (int i = threadIdx.x; i < T2; i += blockDim.x) {
+      const uint32_t v = s_tile[i];
+      if (!v) continue;
+      const int pi = a.tile_i0 + i / T, pj = a.tile_j0 + i % T;
+      atomicAdd(&g0.prf[(size_t)pi * g0.ydim + pj], (unsigned long long)v);
+    }
+  }
+  for (int gi = threadIdx.x; gi < a.n_geoms; gi += blockDim.x) {
+    unsigned long long* c = a.counters + (size_t)gi * FD2_NCOUNT;
+    if (s_cnt[gi * 3 + 0]) atomicAdd(&c[FD2_C_GENERATED], (unsigned long long)s_cnt[gi * 3 + 0]);
+    if (s_cnt[gi * 3 + 1]) atomicAdd(&c[FD2_C_DETECT], (unsigned long long)s_cnt[gi * 3 + 1]);
+    if (s_cnt[gi * 3 + 2])
+      atomicAdd(&c[FD2_C_SURR_UNDETECTED], (unsigned long long)s_cnt[gi * 3 + 2]);
+  }
+}
+
+uint32_t mix32(uint32_t x) {  // integer hash (murmur3 finaliser)
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  x *= 0xC2B2AE35u;
+  x ^= x >> 16;
+  return x;
+}
+
+}  // namespace fd2mlp
+
+extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events,
+                                    int64_t n, const fd2_surrogate_cfg* cfg_in,
+                                    fd2_surrogate_stats* st) {
+  if (!c || !p || n < 0 || (n > 0 && !events)) return FD2_E_INVAL;
+  if (c->device != p->device) return FD2_E_INVAL;
+  fd2_surrogate_cfg cfg;
+  if (cfg_in) {
+    cfg = *cfg_in;
+  } else {
+    std::memset(&cfg, 0, sizeof cfg);
+    cfg.surface_band = 5.0f;
+    cfg.audit_fraction = 0.01f;
+    cfg.h_min = 100.0f; cfg.h_max = 500.0f;
+    cfg.r_min = 2.55f; cfg.r_max = 5.0f;
+    cfg.refl_min = 0.5f; cfg.refl_max = 0.95f;
+  }
+  fd2_surrogate_stats stats;
+  std::memset(&stats, 0, sizeof stats);
+  const auto t0 = std::chrono::steady_clock::now();
+  MLP_HIP(hipSetDevice(c->device));
+  for (int64_t i = 0; i < n; ++i) {
+    if (events[i].N < 0 || events[i].N > FD2_MAX_PHOTONS_PER_EVENT) return FD2_E_TOO_MANY;
+    if (events[i].geom >= (uint32_t)c->n_geoms) return FD2_E_INVAL;
+  }
+  // routing
+  bool in_range[FD2_MAX_GEOMS];
+  for (int g = 0; g < c->n_geoms; ++g) {
+    const fd2_params& q = c->params[g];
+    const double refl = 1.0 - q.dettop;
+    in_range[g] = q.detheight >= cfg.h_min && q.detheight <= cfg.h_max &&
+                  q.detradius >= cfg.r_min && q.detradius <= cfg.r_max &&
+                  refl >= cfg.refl_min && refl <= cfg.refl_max;
+  }
+  std::vector<fd2_event> mc, audit, surr;
+  std::vector<uint32_t> surr_ids;
+  const uint64_t audit_cut = (uint64_t)((double)cfg.audit_fraction * 4294967296.0);
+  for (int64_t i = 0; i < n; ++i) {
+    const fd2_event& e = events[i];
+    const fd2::GeomDev& g = c->hgeom[e.geom];
+    const uint32_t gid = (uint32_t)(c->next_event + i);
+    const bool near = e.z > g.half_h - cfg.surface_band || e.z < -g.half_h + cfg.surface_band;
+    if (near || !in_range[e.geom]) {
+      mc.push_back(e);
+      stats.photons_mc += e.N;
+    } else if ((uint64_t)fd2mlp::mix32(gid ^ fd2mlp::mix32(cfg.seed)) < audit_cut) {
+      audit.push_back(e);
+      stats.photons_mc += e.N;
+    } else {
+      surr.push_back(e);
+      surr_ids.push_back(gid);
+      stats.photons_surrogate += e.N;
+    }
+  }
+  stats.n_events = n;
+  stats.n_mc = (int64_t)mc.size();
+  stats.n_audit = (int64_t)audit.size();
+  stats.n_surrogate = (int64_t)surr.size();
+  const int64_t base = c->next_event;
+
+  // Monte Carlo part (asynchronous on the context's stream)
+  int rc = FD2_OK;
+  const auto tm0 = std::chrono::steady_clock::now();
+  if (!mc.empty()) rc = fd2_submit(c, mc.data(), (int64_t)mc.size());
+  if (rc) return rc;
+  // audit slice: Monte Carlo with per-event radial tallies, and the
+  // network's prediction for the same events
+  if (!audit.empty()) {
+    std::vector<uint32_t> rad(audit.size() * FD2_RADIAL_BINS);
+    rc = fd2_submit_radial(c, audit.data(), (int64_t)audit.size(), rad.data());
+    if (rc) return rc;
+    std::vector<float> feat(audit.size() * kFeat), pred(audit.size() * kTarget);
+    for (size_t i = 0; i < audit.size(); ++i) {
+      const fd2_event& e = audit[i];
+      const fd2::GeomDev& g = c->hgeom[e.geom];
+      feat[i * kFeat + 0] = (g.half_h - e.z) / g.H;
+      feat[i * kFeat + 1] = e.col_dx / g.R;
+      feat[i * kFeat + 2] = g.H * (1.0f / 500.0f);
+      feat[i * kFeat + 3] = g.R * (1.0f / 5.0f);
+      feat[i * kFeat + 4] = 1.0f - g.top_absfrac;
+    }
+    rc = fd2_mlp_forward(p, feat.data(), (int64_t)audit.size(), pred.data());
+    if (rc) return rc;
+    std::vector<double> pm(FD2_RADIAL_BINS, 0.0), pp(FD2_RADIAL_BINS, 0.0);
+    for (size_t i = 0; i < audit.size(); ++i) {
+      double det = 0.0;
+      for (int b = 0; b < FD2_RADIAL_BINS; ++b) {
+        det += rad[i * FD2_RADIAL_BINS + b];
+        pm[b] += rad[i * FD2_RADIAL_BINS + b];
+      }
+      const double nd = (double)audit[i].N * pred[i * kTarget];
+      stats.audit_detected_mc += det;
+      stats.audit_detected_pred += nd;
+      for (int b = 0; b < FD2_RADIAL_BINS; ++b) pp[b] += nd * pred[i * kTarget + 1 + b];
+    }
+    double sm = 0, sp = 0, l1 = 0;
+    for (int b = 0; b < FD2_RADIAL_BINS; ++b) {
+      sm += pm[b];
+      sp += pp[b];
+    }
+    for (int b = 0; b < FD2_RADIAL_BINS; ++b)
+      l1 += std::fabs((sm > 0 ? pm[b] / sm : 0) - (sp > 0 ? pp[b] / sp : 0));
+    stats.audit_radial_l1 = l1;
+  }
+  MLP_HIP(hipStreamSynchronize(c->stream));
+  const auto tm1 = std::chrono::steady_clock::now();
+
+  // network part
+  if (!surr.empty()) {
+    const int64_t ns = (int64_t)surr.size();
+    const int chunk = 65536;
+    rc = ensure_batch(p, chunk);
+    if (rc) return rc;
+    fd2_event* d_ev = nullptr;
+    uint32_t* d_ids = nullptr;
+    MLP_HIP(hipMalloc(&d_ev, (size_t)ns * sizeof(fd2_event)));
+    if (hipMalloc(&d_ids, (size_t)ns * 4) != hipSuccess) {
+      (void)hipFree(d_ev);
+      return FD2_E_NOMEM;
+    }
+    int tile = c->tune.lds_tile > 0 ? c->tune.lds_tile : kTileDefault;
+    const fd2::GeomDev& g0 = c->hgeom[0];
+    if (tile > g0.xdim) tile = g0.xdim;
+    if (tile > g0.ydim) tile = g0.ydim;
+    int dev_cu = c->n_cu > 0 ? c->n_cu : 1;
+    if (hipMemcpyAsync(d_ev, surr.data(), (size_t)ns * sizeof(fd2_event), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess ||
+        hipMemcpyAsync(d_ids, surr_ids.data(), (size_t)ns * 4, hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+      rc = FD2_E_DEVICE;
+    for (int64_t s0 = 0; s0 < ns && rc == FD2_OK; s0 += chunk) {
+      const int b = (int)(ns - s0 < chunk ? ns - s0 : chunk);
+      hipLaunchKernelGGL(fd2mlp::k_surr_features, dim3((b * kInPad + 255) / 256), dim3(256), 0,
+                         c->stream, d_ev + s0, b, c->d_geoms, p->x0);
+      rc = forward(p, b, c->stream);
+      if (rc) break;
+      fd2mlp::SampleArgs a;
+      a.events = d_ev + s0;
+      a.ids = d_ids + s0;
+      a.logits = p->z[3];
+      a.geoms = c->d_geoms;
+      a.counters = c->d_counters;
+      a.n = b;
+      a.n_geoms = c->n_geoms;
+      a.seed = cfg.seed;
+      a.tile = tile;
+      a.tile_i0 = g0.xdim / 2 - tile / 2;
+      a.tile_j0 = g0.ydim / 2 - tile / 2;
+      int blocks = dev_cu * 4;
+      if (blocks * 4 > b) blocks = (b + 3) / 4;
+      hipLaunchKernelGGL(fd2mlp::k_surr_sample, dim3(blocks), dim3(fd2mlp::kSampleThreads),
+                         (size_t)tile * tile * 4, c->stream, a);
+      if (hipGetLastError() != hipSuccess) rc = FD2_E_DEVICE;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = FD2_E_DEVICE;
+    (void)hipFree(d_ev);
+    (void)hipFree(d_ids);
+    if (rc) return rc;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  c->next_event = base + n > c->next_event ? base + n : c->next_event;
+  stats.ms_mc = std::chrono::duration<double, std::milli>(tm1 - tm0).count();
+  stats.ms_surrogate = std::chrono::duration<double, std::milli>(t1 - tm1).count();
+  stats.ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (st) *st = stats;
+  return FD2_OK;
+}

@@ -26,7 +26,7 @@ import numpy as np
 
 EVENT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("E", "<f4"),
                         ("hist", "<i4"), ("N", "<i4"), ("geom", "<u4"),
-                        ("flags", "<u4")])
+                        ("col_dx", "<f4")])
 
 # Reference parameter defaults (main/hybridMANTIS_input.in:8-31) adjusted to
 # the workloads below: 1001 um detector with the beam at its centre, 501x501

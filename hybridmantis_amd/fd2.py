@@ -26,7 +26,7 @@ STEP_DTYPE = np.dtype([("pos_in", "<f4", 3), ("dcos", "<f4", 3), ("xc", "<f4"),
                        ("yc", "<f4"), ("pos_out", "<f4", 3), ("code", "<i4"),
                        ("photon", "<i4"), ("decisions", "<u4")])
 COUNTER_NAMES = ["generated", "detected", "abs_top", "abs_bulk", "lost",
-                 "outofcol", "theta1", "regenerated"]
+                 "outofcol", "theta1", "regenerated", "surr_undetected"]
 
 FD2_OK = 0
 
@@ -276,7 +276,7 @@ class Transport:
         return out
 
     def counters(self, geom: int = 0) -> dict:
-        out = np.zeros(8, np.uint64)
+        out = np.zeros(len(COUNTER_NAMES), np.uint64)
         _check(lib().fd2_get_counters(self._ctx, geom, out.ctypes.data), "get_counters")
         return dict(zip(COUNTER_NAMES, (int(v) for v in out)))
 
@@ -287,3 +287,155 @@ class Transport:
         ms = C.c_float()
         _check(lib().fd2_last_kernel_ms(self._ctx, C.byref(ms)), "last_kernel_ms")
         return float(ms.value)
+
+
+_mlp_sigs_done = False
+
+
+def _mlp_lib() -> C.CDLL:
+    global _mlp_sigs_done
+    L = lib()
+    if not _mlp_sigs_done:
+        P = C.c_void_p
+        sig = {
+            "fd2_mlp_num_params": ([], C.c_size_t),
+            "fd2_mlp_create": ([C.c_int, C.c_uint32, C.POINTER(P)], C.c_int),
+            "fd2_mlp_destroy": ([P], None),
+            "fd2_mlp_set_params": ([P, P, C.c_size_t], C.c_int),
+            "fd2_mlp_get_params": ([P, P, C.c_size_t], C.c_int),
+            "fd2_mlp_forward": ([P, P, C.c_int64, P], C.c_int),
+            "fd2_mlp_gradients": ([P, P, P, C.c_int, P, C.POINTER(C.c_float)], C.c_int),
+            "fd2_mlp_set_data": ([P, P, P, C.c_int64], C.c_int),
+            "fd2_mlp_train": ([P, C.c_int, C.c_int, C.c_float, C.c_uint32, P,
+                               C.POINTER(C.c_float)], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _mlp_sigs_done = True
+    return L
+
+
+class Mlp:
+    """The surrogate network on the GPU (BF16 matrix cores, FP32 master
+    weights): 5 features -> [detection probability, 64 radial bins]."""
+
+    FEATURES = 5
+    TARGETS = 65
+
+    def __init__(self, device: int = 0, seed: int = 0):
+        L = _mlp_lib()
+        h = C.c_void_p()
+        _check(L.fd2_mlp_create(device, seed & 0xFFFFFFFF, C.byref(h)), "fd2_mlp_create")
+        self._h = h
+        self.n_params = int(L.fd2_mlp_num_params())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _mlp_lib().fd2_mlp_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        _check(_mlp_lib().fd2_mlp_get_params(self._h, out.ctypes.data, self.n_params), "get_params")
+        return out
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, np.float32)
+        _check(_mlp_lib().fd2_mlp_set_params(self._h, a.ctypes.data, a.size), "set_params")
+
+    def forward(self, x) -> np.ndarray:
+        a = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((len(a), self.TARGETS), np.float32)
+        _check(_mlp_lib().fd2_mlp_forward(self._h, a.ctypes.data, len(a), out.ctypes.data),
+               "mlp_forward")
+        return out
+
+    def gradients(self, x, y):
+        a = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        _check(_mlp_lib().fd2_mlp_gradients(self._h, a.ctypes.data, t.ctypes.data, len(a),
+                                            g.ctypes.data, C.byref(loss)), "mlp_gradients")
+        return float(loss.value), g
+
+    def set_data(self, x, y) -> None:
+        self._data_x = np.ascontiguousarray(x, np.float32)
+        self._data_y = np.ascontiguousarray(y, np.float32)
+        _check(_mlp_lib().fd2_mlp_set_data(self._h, self._data_x.ctypes.data,
+                                           self._data_y.ctypes.data, len(self._data_x)),
+               "mlp_set_data")
+
+    def train(self, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0):
+        losses = np.zeros(steps, np.float32)
+        ms = C.c_float()
+        _check(_mlp_lib().fd2_mlp_train(self._h, steps, batch, lr, seed & 0xFFFFFFFF,
+                                        losses.ctypes.data, C.byref(ms)), "mlp_train")
+        return losses, float(ms.value)
+
+
+class SurrogateCfg(C.Structure):
+    _fields_ = [("surface_band", C.c_float), ("audit_fraction", C.c_float),
+                ("h_min", C.c_float), ("h_max", C.c_float),
+                ("r_min", C.c_float), ("r_max", C.c_float),
+                ("refl_min", C.c_float), ("refl_max", C.c_float),
+                ("seed", C.c_uint32), ("reserved", C.c_int32)]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("n_events", "n_mc", "n_surrogate", "n_audit",
+                                         "photons_mc", "photons_surrogate")] + \
+               [(n, C.c_double) for n in ("audit_detected_mc", "audit_detected_pred",
+                                          "audit_radial_l1", "ms_total", "ms_mc",
+                                          "ms_surrogate")]
+
+
+SURROGATE_DEFAULTS = dict(surface_band=5.0, audit_fraction=0.01, h_min=100.0, h_max=500.0,
+                          r_min=2.55, r_max=5.0, refl_min=0.5, refl_max=0.95, seed=0)
+
+
+def surrogate_submit(tr: "Transport", net: "Mlp", events, **cfg) -> dict:
+    """Transport `events` with the surrogate where it applies and the plain
+    Monte Carlo elsewhere (surface band, out-of-range geometry, audit
+    slice); tallies accumulate in `tr`.  Returns the routing and audit
+    statistics."""
+    L = lib()
+    f = L.fd2_surrogate_submit
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                  C.POINTER(SurrogateCfg), C.POINTER(SurrogateStats)]
+    f.restype = C.c_int
+    c = dict(SURROGATE_DEFAULTS)
+    c.update(cfg)
+    sc = SurrogateCfg(**c)
+    st = SurrogateStats()
+    ev = as_events(events)
+    _check(f(tr._ctx, net._h, ev.ctypes.data, len(ev), C.byref(sc), C.byref(st)),
+           "fd2_surrogate_submit")
+    return {k: getattr(st, k) for k, _ in SurrogateStats._fields_}
+
+
+def submit_radial(tr: "Transport", events) -> np.ndarray:
+    """Transport events and return each event's 64-bin log-radial histogram
+    of detection offsets (u32 [n, 64])."""
+    L = lib()
+    f = L.fd2_submit_radial
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    f.restype = C.c_int
+    ev = as_events(events)
+    out = np.zeros((len(ev), 64), np.uint32)
+    _check(f(tr._ctx, ev.ctypes.data, len(ev), out.ctypes.data), "fd2_submit_radial")
+    return out

@@ -35,7 +35,10 @@ extern "C" {
 /* largest PRF the reference's COMMON /outputimage/ holds (501x501) is not a
  * limit here; a PRF may have up to this many pixels per side */
 #define FD2_MAX_PRF_SIDE 4096
-#define FD2_MAX_GEOMS 16
+#define FD2_MAX_GEOMS 32
+/* log-radial tally of detection offsets from the event: bin 0 = [0, 0.5) um,
+ * bins 1..62 log-spaced over [0.5, 200) um, bin 63 = [200, inf) um */
+#define FD2_RADIAL_BINS 64
 
 /* Detector and tally parameters: the reference's COMMON /inputargs/ block,
  * same members, order and types (main/kernel_cuda_c_ver1_0.cu:178-205,
@@ -71,7 +74,10 @@ typedef struct fd2_event {
   int32_t hist;
   int32_t N;
   uint32_t geom;
-  uint32_t flags;           /* reserved, 0                                  */
+  /* x offset (um) of the event from the axis of its first column.  The
+   * reference always centres the first column on the event (0); non-zero
+   * values serve the surrogate's training data only. */
+  float col_dx;
 } fd2_event;
 
 /* Per-photon record written by the recording entry points. */
@@ -91,7 +97,9 @@ enum {
  * (main/kernel_cuda_c_ver1_0.cu:111-121) plus trapped-photon restarts */
 enum {
   FD2_C_GENERATED = 0, FD2_C_DETECT, FD2_C_ABS_TOP, FD2_C_ABS_BULK,
-  FD2_C_LOST, FD2_C_OUTOFCOL, FD2_C_THETA1, FD2_C_REGEN, FD2_NCOUNT
+  FD2_C_LOST, FD2_C_OUTOFCOL, FD2_C_THETA1, FD2_C_REGEN,
+  FD2_C_SURR_UNDETECTED,  /* photons of surrogate events not detected       */
+  FD2_NCOUNT
 };
 
 /* Launch shape of the persistent transport kernel. 0 = library default. */
@@ -131,6 +139,12 @@ int fd2_sync(fd2_ctx *ctx);
  * (Σ N records, events in order, photons in order). */
 int fd2_submit_record(fd2_ctx *ctx, const fd2_event *events, int64_t n,
                       fd2_photon *records);
+
+/* Same as fd2_submit but synchronous, and writes for each event the
+ * log-radial histogram of its detections' offsets from the event position
+ * (FD2_RADIAL_BINS u32 per event, events in order). */
+int fd2_submit_radial(fd2_ctx *ctx, const fd2_event *events, int64_t n,
+                      uint32_t *radial);
 
 /* Trajectory-test mode: one GPU lane per event, photons in sequence, one
  * RANECU stream per event seeded as the reference C driver does
@@ -186,6 +200,64 @@ int fd2_phs_from_counts(const uint32_t *hist, int32_t num_hist, int32_t min_ph,
                         int32_t max_ph, int32_t num_bins, int32_t *phs);
 
 const char *fd2_strerror(int status);
+
+/* ---- surrogate network ------------------------------------------------- */
+/* MLP 5 -> 256 -> 256 -> 256 -> 65 with GELU, BF16 weights and activations
+ * on the matrix cores with FP32 accumulation, FP32 master weights, Adam.
+ * Features (FD2_MLP_FEATURES per sample): depth below the top surface / H,
+ * col_dx / R, H / 500 um, R / 5 um, top reflectivity (1 - dettop).
+ * Outputs/targets (FD2_MLP_TARGETS): detection probability, then the
+ * FD2_RADIAL_BINS-bin distribution of detection offsets.  No counterpart in
+ * the reference (the surrogate is an addition to its transport). */
+#define FD2_MLP_FEATURES 5
+#define FD2_MLP_TARGETS 65
+typedef struct fd2_mlp fd2_mlp;
+size_t fd2_mlp_num_params(void);
+int fd2_mlp_create(int device, uint32_t seed, fd2_mlp **out);
+void fd2_mlp_destroy(fd2_mlp *mlp);
+int fd2_mlp_set_params(fd2_mlp *mlp, const float *params, size_t n);
+int fd2_mlp_get_params(fd2_mlp *mlp, float *params, size_t n);
+/* probabilities for n host samples: out[n][FD2_MLP_TARGETS] */
+int fd2_mlp_forward(fd2_mlp *mlp, const float *x, int64_t n, float *out);
+/* gradient of the mean loss over `batch` samples (batch % 16 == 0), no
+ * update: grads[fd2_mlp_num_params()], *loss */
+int fd2_mlp_gradients(fd2_mlp *mlp, const float *x, const float *y, int batch,
+                      float *grads, float *loss);
+/* keep a training set resident on the GPU, then run Adam steps on random
+ * minibatches of it; losses[steps] and *ms (GPU time) may be NULL */
+int fd2_mlp_set_data(fd2_mlp *mlp, const float *x, const float *y, int64_t n);
+int fd2_mlp_train(fd2_mlp *mlp, int steps, int batch, float lr, uint32_t seed,
+                  float *losses, float *ms);
+
+/* Surrogate transport with plain-MC fallback.  Each event goes to the plain
+ * Monte Carlo when it lies within surface_band of the top or bottom surface,
+ * when its geometry is outside the network's training ranges, or (audit)
+ * with probability audit_fraction; all other events are transported by the
+ * network: per event a binomial number of detected photons and, for each,
+ * a radius drawn from the predicted radial distribution and a uniform
+ * azimuth, tallied like the Monte Carlo's detections.  Audit events are
+ * tallied from the Monte Carlo and also compared with the network's
+ * prediction (fd2_surrogate_stats).  Synchronous. */
+typedef struct fd2_surrogate_cfg {
+  float surface_band;       /* um (default 5)                               */
+  float audit_fraction;     /* (default 0.01)                               */
+  float h_min, h_max;       /* training ranges of thickness,                */
+  float r_min, r_max;       /* column radius                                */
+  float refl_min, refl_max; /* and top reflectivity                         */
+  uint32_t seed;
+  int32_t reserved;
+} fd2_surrogate_cfg;
+typedef struct fd2_surrogate_stats {
+  int64_t n_events, n_mc, n_surrogate, n_audit;
+  int64_t photons_mc, photons_surrogate;
+  double audit_detected_mc;      /* detections of audit events (MC)         */
+  double audit_detected_pred;    /* sum over audit events of N * p_det      */
+  double audit_radial_l1;        /* L1 of pooled radial distributions       */
+  double ms_total, ms_mc, ms_surrogate;
+} fd2_surrogate_stats;
+int fd2_surrogate_submit(fd2_ctx *ctx, fd2_mlp *mlp, const fd2_event *events,
+                         int64_t n, const fd2_surrogate_cfg *cfg,
+                         fd2_surrogate_stats *stats);
 
 /* ---- reference file formats -------------------------------------------- */
 

@@ -340,8 +340,8 @@ static float dist_to_surface(ctx *c) {
   const float R = g->R, H = g->H;
   c->t->work[FDO_W_DIST]++;
   float t0 = p[0], t1 = p[1], t2 = p[2];
-  if (c->nreb == 0) {
-    c->xc = c->ev->x;
+  if (c->nreb == 0) { /* first column on the event (K:985-989) */
+    c->xc = c->ev->col_dx == 0.0f ? c->ev->x : c->ev->x - c->ev->col_dx;
     c->yc = c->ev->y;
   }
   const float xc = c->xc, yc = c->yc;

@@ -92,6 +92,7 @@ typedef struct {      /* one energy-deposition event, positions in um     */
   float E;            /* eV (unused by the transport)                      */
   int32_t hist;       /* 1-based x-ray history number                      */
   int32_t N;          /* optical photons to transport                      */
+  float col_dx;       /* event x - first column axis x (0 in the reference) */
 } fdo_event;
 
 typedef struct {      /* per-photon record (optional output)               */

@@ -40,7 +40,7 @@ class Geom(C.Structure):
 
 
 EVENT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("E", "<f4"),
-                        ("hist", "<i4"), ("N", "<i4")])
+                        ("hist", "<i4"), ("N", "<i4"), ("col_dx", "<f4")])
 PHOTON_DTYPE = np.dtype([("fate", "<i4"), ("pix", "<i4"), ("x", "<f4"),
                          ("y", "<f4")])
 
@@ -141,7 +141,8 @@ def _as_events(ev) -> np.ndarray:
     if a.dtype != EVENT_DTYPE:
         out = np.zeros(len(a), EVENT_DTYPE)
         for k in EVENT_DTYPE.names:
-            out[k] = a[k]
+            if k in a.dtype.names:
+                out[k] = a[k]
         a = out
     return a
 

@@ -1,0 +1,141 @@
+"""Surrogate network kernels (BF16 MFMA GEMMs, GELU, heads, Adam) against
+the float64 CPU reference with the kernels' bf16 rounding points."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+def batch(rng, B):
+    x = rng.uniform(0, 1, (B, 5)).astype(np.float32)
+    y = np.zeros((B, 65), np.float32)
+    y[:, 0] = rng.uniform(0.05, 0.95, B)
+    y[:, 1:] = rng.dirichlet(np.ones(64) * 0.5, B)
+    y[::7, 1:] = 0.0  # samples without detections
+    return x, y
+
+
+def test_forward_matches_reference(fd2lib):
+    rng = np.random.default_rng(0)
+    p = M.init_params(5)
+    p[M.B_OFF[2]:M.B_OFF[2] + 256] = rng.normal(0, 0.2, 256)
+    x, _ = batch(rng, 300)
+    with fd2lib.Mlp() as net:
+        net.set_params(p)
+        out = net.forward(x)
+    ref = M.probs(p, x)
+    assert np.abs(out - ref).max() < 2e-3
+    assert np.allclose(out[:, 1:].sum(1), 1.0, atol=1e-5)
+
+
+def test_gradients_match_reference(fd2lib):
+    rng = np.random.default_rng(1)
+    p = M.init_params(7)
+    x, y = batch(rng, 512)
+    with fd2lib.Mlp() as net:
+        net.set_params(p)
+        loss, g = net.gradients(x, y)
+        assert np.array_equal(net.params(), p)  # no update
+    rl, rg = M.loss_and_grads(p, x, y, rnd=True)
+    assert abs(loss - rl) < 1e-4 * max(1.0, abs(rl))
+    for l in range(4):
+        for off, size in ((M.W_OFF[l], M.SHAPES[l][0] * M.SHAPES[l][1]),
+                          (M.B_OFF[l], M.SHAPES[l][1])):
+            a, b = g[off:off + size].astype(np.float64), rg[off:off + size]
+            err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+            assert err < 2e-2, (l, off, err)
+    cos = g.astype(np.float64) @ rg / (np.linalg.norm(g) * np.linalg.norm(rg))
+    assert cos > 0.9999
+
+
+def test_training_reduces_loss(fd2lib):
+    rng = np.random.default_rng(2)
+    n = 20000
+    x = rng.uniform(0, 1, (n, 5)).astype(np.float32)
+    y = np.zeros((n, 65), np.float32)
+    y[:, 0] = 0.2 + 0.6 * x[:, 0]  # smooth target
+    centre = 8 + 40 * x[:, 0]
+    k = np.arange(64)
+    q = np.exp(-0.5 * ((k[None, :] - centre[:, None]) / 4.0) ** 2)
+    y[:, 1:] = q / q.sum(1, keepdims=True)
+    with fd2lib.Mlp(seed=3) as net:
+        net.set_data(x, y)
+        losses, ms = net.train(300, batch=1024, lr=2e-3)
+        pred = net.forward(x[:2000])
+    # the loss cannot go below the targets' own entropy (BCE + CE at q = y)
+    y0 = y[:, 0].astype(np.float64)
+    q = y[:, 1:].astype(np.float64)
+    floor = np.mean(-(y0 * np.log(y0) + (1 - y0) * np.log(1 - y0))) + \
+        np.mean(-(q * np.log(np.maximum(q, 1e-30))).sum(1))
+    first, last = losses[:5].mean(), losses[-20:].mean()
+    assert last - floor < 0.1 * (first - floor), (first, last, floor)
+    assert np.abs(pred[:, 0] - y[:2000, 0]).mean() < 0.03
+    assert np.abs(pred[:, 1:] - y[:2000, 1:]).sum(1).mean() < 0.15
+    assert ms > 0
+
+
+def test_radial_tally_matches_detection_points(fd2lib):
+    from hybridmantis_amd import events as E, surrogate as S
+
+    p = E.params(num_hist=10)
+    ev = E.generate(10, 150.0, 25.0, 150.0, 5)
+    with fd2lib.Transport(p) as tr:
+        rad = fd2lib.submit_radial(tr, ev)
+        tr.reset()
+        rec = tr.submit_record(ev)
+    off = np.concatenate([[0], np.cumsum(ev["N"])])
+    edges = S.radial_edges()
+    mism = 0
+    for e in range(len(ev)):
+        r = rec[off[e]:off[e + 1]]
+        det = (r["fate"] == 1) | (r["fate"] == 2)
+        d = np.hypot(r["x"][det].astype(np.float64) - ev["x"][e], r["y"][det] - ev["y"][e])
+        h = np.histogram(d, bins=edges[:-1].tolist() + [np.inf])[0]
+        mism += np.abs(h - rad[e]).sum()
+        assert rad[e].sum() == det.sum()
+    assert mism <= 0.002 * rad.sum()  # bin-edge rounding only
+
+
+def test_first_column_offset_is_an_extension(fd2lib, oracle):
+    from hybridmantis_amd import events as E
+
+    p = E.params(num_hist=4)
+    ev = E.generate(4, 150.0, 25.0, 150.0, 6)
+    ev["col_dx"] = 1.7
+    with fd2lib.Transport(p) as tr:
+        rec = tr.submit_record(ev)
+    o = oracle.run(p, ev, 4, rng=oracle.RNG_COUNTER, math=oracle.MATH_DEVICE, record=True)
+    assert np.array_equal(rec["fate"], o.photons["fate"])
+    assert np.array_equal(rec["pix"], o.photons["pix"])
+
+
+def test_surrogate_pipeline_small(fd2lib):
+    """Tiny training set, then surrogate + fallback against the plain MC on
+    the same events: routing, conservation and a loose accuracy check."""
+    from hybridmantis_amd import events as E, surrogate as S, stats
+
+    x, y, didx = S.generate(n_depth=24, n_offset=1, photons=4000, log=None)
+    net, rep = S.train(x, y, didx, steps=400, batch=1024, lr=3e-3, log=None)
+    assert rep["holdout_pdet_mae"] < 0.05
+    p, ev = E.workload("mono50", scale=0.002)
+    n = int(ev["N"].sum())
+    with fd2lib.Transport(p) as tr:
+        st = fd2lib.surrogate_submit(tr, net, ev, audit_fraction=0.05)
+        c = tr.counters()
+        prf, hist = tr.prf(), tr.hist_counts()
+    with fd2lib.Transport(p) as tr:
+        tr.submit(ev)
+        c_mc, prf_mc = tr.counters(), tr.prf()
+    net.close()
+    assert st["n_mc"] + st["n_surrogate"] + st["n_audit"] == len(ev)
+    assert st["photons_mc"] + st["photons_surrogate"] == n
+    assert st["n_surrogate"] > 0.8 * len(ev)
+    assert c["generated"] == n
+    assert c["detected"] + c["abs_top"] + c["abs_bulk"] + c["lost"] + c["outofcol"] + \
+        c["theta1"] + c["surr_undetected"] == n
+    assert int(hist.sum()) == c["detected"]
+    assert int(prf.sum()) <= c["detected"]
+    assert abs(c["detected"] / c_mc["detected"] - 1) < 0.1
+    assert stats.l1_distance(prf, prf_mc) < 0.5
