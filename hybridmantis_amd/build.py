@@ -1,4 +1,5 @@
-"""Build libfd2.so (HIP kernels + C host library) in-tree for gfx950.
+"""Build libfd2.so (HIP kernels + C host library) and libfd2_fortran.so (the
+gfortran-ABI entry points) in-tree for gfx950.
 
     python -m hybridmantis_amd.build          # or hybridmantis_amd.build.build()
 
@@ -18,7 +19,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfd2.so")
 
 HIP_SOURCES = ["fd2_kernels.hip", "fd2_host.cpp", "fd2_mlp.hip"]
-CXX_SOURCES = ["fd2_io.cpp", "fd2_fortran.cpp"]
+CXX_SOURCES = ["fd2_io.cpp"]
+FORTRAN_SOURCE = "fd2_fortran.cpp"
+FORTRAN_LIB = os.path.join(HERE, "libfd2_fortran.so")
 
 ARCH = os.environ.get("FD2_OFFLOAD_ARCH", "gfx950")
 
@@ -57,7 +60,7 @@ def build(verbose: bool = False, jobs: int = 4, out: str | None = None,
         if src in HIP_SOURCES:
             cmd = [hipcc, "-x", "hip", *HIP_FLAGS, *[f"-D{d}" for d in defines],
                    "-c", path, "-o", obj]
-        else:  # host-only C++ (file formats, Fortran-ABI shim)
+        else:  # host-only C++ (file formats)
             cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC",
                    "-Wall", "-I", os.path.join(ROOT, "include"), "-c", path,
                    "-o", obj]
@@ -73,9 +76,9 @@ def build(verbose: bool = False, jobs: int = 4, out: str | None = None,
             procs[0].wait()
     failed = []
     for cmd, p in zip(cmds, procs):
-        out, _ = p.communicate()
-        if out and (verbose or p.returncode):
-            sys.stderr.write(out)
+        log, _ = p.communicate()
+        if log and (verbose or p.returncode):
+            sys.stderr.write(log)
         if p.returncode:
             failed.append(cmd[-3])
     if failed:
@@ -86,7 +89,26 @@ def build(verbose: bool = False, jobs: int = 4, out: str | None = None,
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
     os.replace(tmp, lib)
+    if out is None:
+        build_fortran(objdir, verbose)
     return lib
+
+
+def build_fortran(objdir: str, verbose: bool = False) -> str:
+    """libfd2_fortran.so: the gfortran-ABI entry points, linked against the
+    libfd2.so beside it."""
+    gxx = shutil.which("g++") or "g++"
+    obj = os.path.join(objdir, "fd2_fortran.o")
+    cmds = [[gxx, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"),
+             "-c", os.path.join(CSRC, FORTRAN_SOURCE), "-o", obj],
+            [gxx, "-shared", "-fPIC", "-o", FORTRAN_LIB + ".tmp", obj, "-L", HERE, "-lfd2",
+             "-Wl,-rpath,$ORIGIN"]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    os.replace(FORTRAN_LIB + ".tmp", FORTRAN_LIB)
+    return FORTRAN_LIB
 
 
 if __name__ == "__main__":

@@ -165,7 +165,39 @@ int fd2_read_ede(const char* path, fd2_event* events, int64_t max, int64_t* n) {
     if (*s == '#' || *s == '\0' || *s == '\n' || *s == '\r') continue;
     double x, y, z, de;
     long h;
-    if (std::sscanf(s, "%lf %lf %lf %lf %ld", &x, &y, &z, &de, &h) != 5) {
+    size_t len = std::strcspn(line, "\r\n");
+    if (len == 72) {
+      // the record's fixed columns: 3 x (1x,1pe16.9), (1x,1pe12.4), i8.  A
+      // history number of 8 digits touches the energy field, so the fields
+      // are cut at their columns rather than at blanks.
+      static const int start[6] = {0, 17, 34, 51, 64, 72};
+      double v[4];
+      bool ok = true;
+      char field[24];
+      for (int f = 0; f < 5 && ok; ++f) {
+        const int w = start[f + 1] - start[f];
+        std::memcpy(field, line + start[f], (size_t)w);
+        field[w] = '\0';
+        char* end = nullptr;
+        errno = 0;
+        if (f < 4) {
+          v[f] = std::strtod(field, &end);
+        } else {
+          h = std::strtol(field, &end, 10);
+        }
+        if (end == field || errno) ok = false;
+        while (ok && end && *end == ' ') ++end;
+        if (ok && end && *end) ok = false;
+      }
+      if (!ok) {
+        rc = FD2_E_IO;
+        break;
+      }
+      x = v[0];
+      y = v[1];
+      z = v[2];
+      de = v[3];
+    } else if (std::sscanf(s, "%lf %lf %lf %lf %ld", &x, &y, &z, &de, &h) != 5) {
       rc = FD2_E_IO;
       break;
     }

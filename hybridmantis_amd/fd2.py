@@ -439,3 +439,74 @@ def submit_radial(tr: "Transport", events) -> np.ndarray:
     out = np.zeros((len(ev), 64), np.uint32)
     _check(f(tr._ctx, ev.ctypes.data, len(ev), out.ctypes.data), "fd2_submit_radial")
     return out
+
+
+# ---- file formats (fd2_io.cpp) ---------------------------------------------------------
+
+def from_params(p: Params) -> dict:
+    """fd2_params -> events.params()-style dictionary."""
+    return {k: getattr(p, v) for k, v in _PARAM_MAP.items()}
+
+
+def _io_lib() -> C.CDLL:
+    L = lib()
+    if not getattr(L, "_io_ready", False):
+        P = C.c_void_p
+        sig = {
+            "fd2_read_input": ([C.c_char_p, C.POINTER(Params), P], C.c_int),
+            "fd2_output_names": ([P, C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t], C.c_int),
+            "fd2_write_prf": ([C.c_char_p, C.POINTER(Params), P, C.c_int32, C.c_int32,
+                               C.c_double], C.c_int),
+            "fd2_write_phs": ([C.c_char_p, P, C.c_int32], C.c_int),
+            "fd2_read_ede": ([C.c_char_p, P, C.c_int64, C.POINTER(C.c_int64)], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        L._io_ready = True
+    return L
+
+
+def read_input(path: str):
+    """hybridMANTIS_input.in -> (parameter dictionary, the ten file-name
+    tokens)."""
+    p = Params()
+    tok = (C.c_char * 160)()
+    _check(_io_lib().fd2_read_input(os.fsencode(path), C.byref(p), tok), f"read {path}")
+    raw = bytes(tok)
+    tokens = [raw[16 * i:16 * i + 16].split(b"\0")[0].decode() for i in range(10)]
+    return from_params(p), tokens
+
+
+def output_names(tokens) -> tuple:
+    """(myimage_..., detected_...) file names from the ten tokens."""
+    tok = (C.c_char * 160)()
+    for i, t in enumerate(tokens):
+        b = t.encode()[:15]
+        C.memmove(C.addressof(tok) + 16 * i, b, len(b))
+    a, b = C.create_string_buffer(512), C.create_string_buffer(512)
+    _check(_io_lib().fd2_output_names(tok, a, 512, b, 512), "output names")
+    return a.value.decode(), b.value.decode()
+
+
+def write_prf(path: str, params: dict, prf, n_hist: float) -> None:
+    prf = np.ascontiguousarray(prf, np.uint64)
+    p = to_params(params)
+    _check(_io_lib().fd2_write_prf(os.fsencode(path), C.byref(p), prf.ctypes.data,
+                                   prf.shape[0], prf.shape[1], float(n_hist)), f"write {path}")
+
+
+def write_phs(path: str, phs) -> None:
+    phs = np.ascontiguousarray(phs, np.int32)
+    _check(_io_lib().fd2_write_phs(os.fsencode(path), phs.ctypes.data, len(phs)),
+           f"write {path}")
+
+
+def read_ede(path: str, max_events: int) -> np.ndarray:
+    """EDE text records -> events (um, eV, history; N = 0)."""
+    ev = np.zeros(max_events, EVENT_DTYPE)
+    n = C.c_int64(0)
+    _check(_io_lib().fd2_read_ede(os.fsencode(path), ev.ctypes.data, max_events, C.byref(n)),
+           f"read {path}")
+    return ev[:n.value]

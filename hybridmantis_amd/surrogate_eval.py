@@ -1,0 +1,117 @@
+"""Surrogate + Monte Carlo fallback against the plain Monte Carlo on the same
+events.
+
+    python -m hybridmantis_amd.surrogate_eval [--scale 1.0] [--noise-floor]
+
+Workload: 1e6 pencil-beam histories at 50 keV on a 500 um layer with 5 um
+columns (events.workload("mono50")).  The plain MC transports every event;
+the surrogate path (fd2_surrogate_submit) sends events within 5 um of the
+top or bottom surface and a 1% audit slice to the MC and draws the rest
+from the network.  Both start from the events in host memory and are timed
+to the tallies being final (wall clock, so routing, copies and launches are
+included).  Reported: histories/s of both and their ratio, the normalised
+PRF L1 distance, the detected-fraction error, and mean detected photons and
+Swank factor of both PHS.  With --noise-floor a second plain-MC run with
+another seed gives the L1 distance two MC runs of this size have anyway.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import events as E
+from . import fd2
+from . import stats
+
+MLP_FWD_FLOP = 2.0 * (5 * 256 + 2 * 256 * 256 + 256 * 65)  # per event
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def _mc(p, ev, seed):
+    with fd2.Transport([p], seed=seed) as tr:
+        t0 = time.perf_counter()
+        tr.submit(ev)
+        tr.sync()
+        dt = time.perf_counter() - t0
+        return dt, tr.prf(0), tr.hist_counts(0), tr.counters(0), tr.last_kernel_ms()
+
+
+def _surrogate(p, ev, net, seed, cfg):
+    with fd2.Transport([p], seed=seed) as tr:
+        t0 = time.perf_counter()
+        st = fd2.surrogate_submit(tr, net, ev, **cfg)
+        dt = time.perf_counter() - t0
+        return dt, tr.prf(0), tr.hist_counts(0), tr.counters(0), st
+
+
+def evaluate(scale: float = 1.0, net=None, seed: int = 271828182, noise_floor: bool = False,
+             log=None, **cfg) -> dict:
+    """Run both paths on workload mono50 at `scale` and compare them."""
+    from . import surrogate as S
+
+    own = net is None
+    net = net or S.load()
+    try:
+        p, ev = E.workload("mono50", scale)
+        n_hist = int(p["num_hist"])
+        warm = ev[: min(len(ev), 4000)]
+        _mc(p, warm, seed)
+        _surrogate(p, warm, net, seed, cfg)
+        t_mc, prf_mc, hist_mc, c_mc, k_ms = _mc(p, ev, seed)
+        if log:
+            log(f"[surrogate-eval] plain MC: {t_mc:.3f} s")
+        t_s, prf_s, hist_s, c_s, st = _surrogate(p, ev, net, seed + 1, cfg)
+        if log:
+            log(f"[surrogate-eval] surrogate + fallback: {t_s:.3f} s")
+        gen = float(c_mc["generated"])
+        det_mc, det_s = c_mc["detected"] / gen, c_s["detected"] / float(c_s["generated"])
+        sw_mc, sw_s = stats.swank(hist_mc), stats.swank(hist_s)
+        md_mc, md_s = stats.mean_detected(hist_mc), stats.mean_detected(hist_s)
+        out = dict(
+            workload="mono50", histories=n_hist, events=int(len(ev)), photons=int(gen),
+            mc_seconds=t_mc, mc_kernel_ms=k_ms, mc_histories_per_s=n_hist / t_mc,
+            mc_photons_per_s=gen / t_mc,
+            surrogate_seconds=t_s, surrogate_histories_per_s=n_hist / t_s,
+            speedup=t_mc / t_s,
+            prf_l1=stats.l1_distance(prf_s, prf_mc),
+            detected_fraction_mc=det_mc, detected_fraction_surrogate=det_s,
+            detected_fraction_rel_err=abs(det_s / det_mc - 1.0),
+            mean_detected_mc=md_mc, mean_detected_surrogate=md_s,
+            mean_detected_rel_err=abs(md_s / md_mc - 1.0),
+            swank_mc=sw_mc, swank_surrogate=sw_s, swank_rel_err=abs(sw_s / sw_mc - 1.0),
+            routing={k: (float(v) if isinstance(v, float) else int(v)) for k, v in st.items()},
+        )
+        if st["ms_surrogate"] > 0:
+            out["mlp_forward_tflops"] = (MLP_FWD_FLOP * st["n_surrogate"] /
+                                         (st["ms_surrogate"] * 1e-3) / 1e12)
+            out["mlp_forward_mfma_frac"] = out["mlp_forward_tflops"] / BF16_PEAK_TFLOPS
+        if noise_floor:
+            _, prf_mc2, hist_mc2, c_mc2, _ = _mc(p, ev, seed + 2)
+            out["prf_l1_mc_vs_mc"] = stats.l1_distance(prf_mc2, prf_mc)
+            out["detected_fraction_mc_vs_mc"] = abs(c_mc2["detected"] / c_mc["detected"] - 1)
+            out["swank_mc_vs_mc"] = abs(stats.swank(hist_mc2) / sw_mc - 1)
+        return out
+    finally:
+        if own:
+            net.close()
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--noise-floor", action="store_true")
+    ap.add_argument("--audit", type=float, default=0.01)
+    ap.add_argument("--band", type=float, default=5.0)
+    a = ap.parse_args(argv)
+    rep = evaluate(a.scale, noise_floor=a.noise_floor, audit_fraction=a.audit,
+                   surface_band=a.band, log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(rep), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -279,8 +279,11 @@ int fd2_write_prf(const char *path, const fd2_params *p, const uint64_t *prf,
 /* PHS writer (tally .f:456-458): mynumbins lines of (1x,I8). */
 int fd2_write_phs(const char *path, const int32_t *phs, int32_t num_bins);
 /* EDE text records (3(1x,1pe16.9),1x,1pe12.4,i8): x y z (cm) de (eV)
- * history (tally .f:173-174).  Reads up to max events, converts cm to um,
- * sets N = 0 (the caller samples the photon count); *n gets the count. */
+ * history (tally .f:173-174).  A 72-character line is cut at the record's
+ * columns; other lines are read as five blank-separated numbers.  Lines
+ * starting with '#' and blank lines are skipped.  Reads up to max events
+ * (FD2_E_INVAL past that), converts cm to um, sets N = 0 (the caller
+ * samples the photon count); *n gets the count. */
 int fd2_read_ede(const char *path, fd2_event *events, int64_t max, int64_t *n);
 
 #ifdef __cplusplus
