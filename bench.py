@@ -14,6 +14,12 @@ per-photon primitive counts measured by the oracle on a slice of the same
 workload) and a CPU baseline: the reference's own C transport branch
 (oracle/_ref, or the oracle restatement when that is not built) run on a
 bounded sample in one process per host core.
+
+Two more legs go into the same line: "batched" (4 geometries x 15-90 keV,
+50,000 histories each, autotuned launch shape, HIP-graph replays) and
+"surrogate" (1e6 histories at 50 keV: surrogate + MC fallback against the
+full plain MC -- histories/s of both, PRF L1, detected-fraction and Swank
+errors).
 """
 from __future__ import annotations
 
@@ -76,6 +82,58 @@ def traffic_from_profile(name: str):
     return float(m.group(1)) if m else None
 
 
+def batched_leg(steps: int) -> dict:
+    """Config-2 batch: 4 geometries x 15-90 keV, 50,000 histories each, in
+    one persistent launch replayed from a HIP graph, with the launch shape
+    picked by the autotuner on every 10th event of the batch."""
+    from hybridmantis_amd import events, fd2
+
+    plist, ev = events.batched_workload()
+    n_ph = int(ev["N"].sum())
+    with fd2.Transport(plist) as tr:
+        buf = tr.upload_events(ev)
+        sample = ev[::10]
+        sbuf = tr.upload_events(sample)
+        best, results = tr.autotune(sbuf, len(sample), repeats=1)
+        sbuf.free()
+        ms = []
+        with tr.graph(buf, len(ev)) as g:
+            g.launch()
+            tr.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                g.launch()
+                tr.sync()
+                ms.append(tr.last_kernel_ms())
+            wall = time.perf_counter() - t0
+        gen = sum(tr.counters(k)["generated"] for k in range(len(plist)))
+        assert gen == n_ph * (steps + 1)
+        buf.free()
+    return {"workload": "4 geometries (H, R) = (100, 2.55), (150, 2.55), (300, 4.0), "
+                        "(500, 5.0) um x uniform 15-90 keV, 50,000 histories each, "
+                        "one HIP-graph launch per step",
+            "events": int(len(ev)), "photons": n_ph, "steps": steps,
+            "photons_per_s": n_ph * steps / wall, "ms_per_launch": sum(ms) / len(ms),
+            "histories_per_s": 4 * 50000 * steps / wall,
+            "tuning": best, "autotune_candidates": len(results)}
+
+
+def surrogate_leg() -> dict:
+    """Config 4: surrogate + MC fallback against the full plain MC on 1e6
+    histories at 50 keV (H = 500 um, R = 5 um)."""
+    from hybridmantis_amd import surrogate_eval
+
+    r = surrogate_eval.evaluate(1.0)
+    keep = ("histories", "events", "photons", "mc_histories_per_s", "surrogate_histories_per_s",
+            "speedup", "prf_l1", "detected_fraction_rel_err", "swank_rel_err",
+            "mean_detected_rel_err", "mlp_forward_tflops", "mlp_forward_mfma_frac")
+    out = {k: r[k] for k in keep if k in r}
+    rt = r["routing"]
+    out["routing"] = {k: rt[k] for k in ("n_events", "n_mc", "n_surrogate", "n_audit")}
+    out["workload"] = "mono50: 1e6 histories at 50 keV, H=500 um, R=5 um"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +143,8 @@ def main() -> int:
     ap.add_argument("--cpu-workers", type=int, default=0)
     ap.add_argument("--cpu-photons", type=int, default=150000,
                     help="photons per CPU process in the baseline sample")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (config 2) leg")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -124,6 +184,8 @@ def main() -> int:
     assert counters["generated"] == n_photons * args.steps
     buf.free()
     tr.close()
+    batched = None if args.no_batched else batched_leg(max(1, min(args.steps, 3)))
+    surr = None if args.no_surrogate else surrogate_leg()
 
     avg_ms = sum(kernel_ms) / len(kernel_ms)
     achieved = fpp * n_photons / (avg_ms * 1e-3) / 1e12
@@ -152,6 +214,8 @@ def main() -> int:
                      "kernel_ms": avg_ms, "flops_per_photon": fpp},
         "cpu_baseline": cpu,
         "histories_per_s": p["num_hist"] * args.steps / wall,
+        "batched": batched,
+        "surrogate": surr,
     }
     print(json.dumps(out), flush=True)
     return 0

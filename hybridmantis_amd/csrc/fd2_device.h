@@ -161,6 +161,8 @@ struct Photon {
 
 // Detection: pixel floor((x-lb)/pitch), inside iff lb <= x <= ub
 // (main/kernel_cuda_c_ver1_0.cu:3402-3419); per-history count at hist-1.
+// With null prf/hist pointers (the persistent kernel's geometry copy) only
+// the fate and pixel are recorded and the caller tallies them.
 __device__ __forceinline__ void tally_detect(Photon& p, const GeomDev& g) {
   p.fate = kDetectOut;
   p.pix = -1;
@@ -171,11 +173,11 @@ __device__ __forceinline__ void tally_detect(Photon& p, const GeomDev& g) {
   if (p.px <= g.ubx && p.py <= g.uby && p.px >= g.lbx && p.py >= g.lby &&
       i >= 0 && i < g.xdim && j >= 0 && j < g.ydim) {
     int idx = i * g.ydim + j;
-    atomicAdd(&g.prf[idx], 1ull);
+    if (g.prf) atomicAdd(&g.prf[idx], 1ull);
     p.fate = kDetect;
     p.pix = idx;
   }
-  if (p.hist >= 1 && p.hist <= g.num_hist) atomicAdd(&g.hist[p.hist - 1], 1u);
+  if (g.hist && p.hist >= 1 && p.hist <= g.num_hist) atomicAdd(&g.hist[p.hist - 1], 1u);
 }
 
 // ------------------------------------------------------- vector helpers

@@ -155,18 +155,44 @@ int validate_host(const fd2_ctx* c, const fd2_event* ev, int64_t n) {
   return FD2_OK;
 }
 
-// queue build + transport on events already in device memory
-int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
-            fd2_photon* d_records, uint32_t* d_radial = nullptr) {
-  if (n <= 0) return FD2_OK;
-  int rc = grow(c, (size_t)n);
-  if (rc) return rc;
+// LDS PRF tiles for a tuning value: -1 none, 0 the default side, else the
+// side; as many geometries get a tile as fit the per-block budget.
+void tile_shape(int lds_tile, int n_geoms, int32_t* side, int32_t* n_tiles) {
+  const int s = lds_tile < 0 ? 0 : (lds_tile == 0 ? kTileDefault : lds_tile);
+  *side = s;
+  if (s == 0) {
+    *n_tiles = 0;
+    return;
+  }
+  const int fit = (s * s > kTileBudget) ? 1 : kTileBudget / (s * s);
+  *n_tiles = n_geoms < fit ? n_geoms : fit;
+}
+
+// Work-queue buffers of one launch (the context's own, or a graph's).
+struct QueueBufs {
+  uint32_t* chunk_off;
+  unsigned long long* photon_off;
+  uint32_t* block_chunks;
+  unsigned long long* block_photons;
+};
+
+// Arguments and grid of the queue build + transport of n device events.
+struct Launch {
   QueueArgs q{};
+  TransportArgs a{};
+  int blocks = 0;
+  bool multi = false, record = false, radial = false;
+};
+
+Launch make_launch(const fd2_ctx* c, const QueueBufs& b, const fd2_event* d_ev, int64_t n,
+                   fd2_photon* d_records, uint32_t* d_radial) {
+  Launch L;
+  QueueArgs& q = L.q;
   q.events = d_ev;
-  q.chunk_off = c->d_chunk_off;
-  q.photon_off = c->d_photon_off;
-  q.block_chunks = c->d_block_chunks;
-  q.block_photons = c->d_block_photons;
+  q.chunk_off = b.chunk_off;
+  q.photon_off = b.photon_off;
+  q.block_chunks = b.block_chunks;
+  q.block_photons = b.block_photons;
   q.cursor = c->d_cursor;
   q.error_flag = c->d_error;
   q.n_events = n;
@@ -174,31 +200,52 @@ int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
   q.chunk_log2 = ilog2(c->tune.chunk);
   q.n_blocks = (int)((n + kScanTile - 1) / kScanTile);
 
-  TransportArgs a{};
+  TransportArgs& a = L.a;
   a.events = d_ev;
-  a.chunk_off = c->d_chunk_off;
-  a.photon_off = c->d_photon_off;
-  a.geoms = c->d_geoms;
+  a.chunk_off = b.chunk_off;
+  a.photon_off = b.photon_off;
+  a.geoms = c->d_geoms_mc;
+  a.tally = c->d_geoms;
   a.counters = c->d_counters;
   a.cursor = c->d_cursor;
   a.records = d_records;
   a.radial = d_radial;
+  a.event_base_dev = nullptr;
   a.n_events = n;
   a.seed = c->seed;
   a.event_base = (uint32_t)c->next_event;
   a.n_geoms = c->n_geoms;
   a.chunk_log2 = q.chunk_log2;
   a.park_lanes = c->tune.park_lanes;
+  a.refill_lanes = c->tune.refill_lanes;
+  tile_shape(c->tune.lds_tile, c->n_geoms, &a.tile_side, &a.n_tiles);
 
-  const bool multi = c->n_geoms > 1;
+  L.multi = c->n_geoms > 1;
+  L.record = d_records != nullptr;
+  L.radial = d_radial != nullptr;
   const int per_cu = c->tune.waves_per_cu / (kTransportThreads / 64);
-  const bool radial = d_radial != nullptr;
-  const int occ = transport_max_blocks_per_cu(multi, record, radial);
-  const int blocks = c->n_cu * (per_cu < 1 ? 1 : (per_cu > occ ? occ : per_cu));
+  const size_t smem = (size_t)a.n_tiles * a.tile_side * a.tile_side * sizeof(uint32_t);
+  const int occ = transport_max_blocks_per_cu(L.multi, L.record, L.radial, smem);
+  L.blocks = c->n_cu * (per_cu < 1 ? 1 : (per_cu > occ ? occ : per_cu));
+  return L;
+}
 
+hipError_t issue(const Launch& L, hipStream_t s) {
+  hipError_t e = launch_queue_build(L.q, s);
+  if (e != hipSuccess) return e;
+  return launch_transport(L.a, L.multi, L.record, L.radial, L.blocks, s);
+}
+
+// queue build + transport on events already in device memory
+int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
+            fd2_photon* d_records, uint32_t* d_radial = nullptr) {
+  if (n <= 0) return FD2_OK;
+  int rc = grow(c, (size_t)n);
+  if (rc) return rc;
+  const QueueBufs b{c->d_chunk_off, c->d_photon_off, c->d_block_chunks, c->d_block_photons};
+  const Launch L = make_launch(c, b, d_ev, n, record ? d_records : nullptr, d_radial);
   FD2_HIP(hipEventRecord(c->ev_start, c->stream));
-  FD2_HIP(launch_queue_build(q, c->stream));
-  FD2_HIP(launch_transport(a, multi, record, radial, blocks, c->stream));
+  FD2_HIP(issue(L, c->stream));
   FD2_HIP(hipEventRecord(c->ev_stop, c->stream));
   c->timed = true;
   c->next_event += n;
@@ -251,6 +298,7 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
   c->tune.waves_per_cu = 16;
   c->tune.lds_tile = 0;
   c->tune.park_lanes = 32;
+  c->tune.refill_lanes = 8;
   int rc = FD2_OK;
   do {
     if (hipSetDevice(device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
@@ -283,8 +331,20 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
       rc = FD2_E_NOMEM;
       break;
     }
-    if (hipMemcpyAsync(c->d_geoms, c->hgeom, sizeof(GeomDev) * n_geoms,
-                       hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+    GeomDev mc[FD2_MAX_GEOMS];
+    for (int i = 0; i < n_geoms; ++i) {
+      mc[i] = c->hgeom[i];
+      mc[i].prf = nullptr;
+      mc[i].hist = nullptr;
+    }
+    if (hipMalloc(&c->d_geoms_mc, sizeof(GeomDev) * n_geoms) != hipSuccess) {
+      rc = FD2_E_NOMEM;
+      break;
+    }
+    if (hipMemcpy(c->d_geoms, c->hgeom, sizeof(GeomDev) * n_geoms, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(c->d_geoms_mc, mc, sizeof(GeomDev) * n_geoms, hipMemcpyHostToDevice) !=
+            hipSuccess) {
       rc = FD2_E_DEVICE;
       break;
     }
@@ -308,7 +368,7 @@ void fd2_destroy(fd2_ctx* c) {
     if (c->d_prf[i]) (void)hipFree(c->d_prf[i]);
     if (c->d_hist[i]) (void)hipFree(c->d_hist[i]);
   }
-  void* bufs[] = {c->d_geoms, c->d_counters, c->d_events, c->d_chunk_off,
+  void* bufs[] = {c->d_geoms, c->d_geoms_mc, c->d_counters, c->d_events, c->d_chunk_off,
                   c->d_photon_off, c->d_block_chunks, c->d_block_photons,
                   c->d_cursor, c->d_error};
   for (void* b : bufs)
@@ -337,11 +397,15 @@ int fd2_set_tuning(fd2_ctx* c, const fd2_tuning* t) {
     if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return FD2_E_INVAL;
     n.waves_per_cu = t->waves_per_cu;
   }
-  if (t->lds_tile < 0) return FD2_E_INVAL;
+  if (t->lds_tile < -1 || t->lds_tile > kTileMax) return FD2_E_INVAL;
   n.lds_tile = t->lds_tile;
   if (t->park_lanes) {
     if (t->park_lanes < 1 || t->park_lanes > 64) return FD2_E_INVAL;
     n.park_lanes = t->park_lanes;
+  }
+  if (t->refill_lanes) {
+    if (t->refill_lanes < 1 || t->refill_lanes > 64) return FD2_E_INVAL;
+    n.refill_lanes = t->refill_lanes;
   }
   c->tune = n;
   return FD2_OK;
@@ -634,6 +698,162 @@ int fd2_last_kernel_ms(fd2_ctx* c, float* ms) {
   FD2_HIP(hipEventSynchronize(c->ev_stop));
   FD2_HIP(hipEventElapsedTime(ms, c->ev_start, c->ev_stop));
   return FD2_OK;
+}
+
+// ---- HIP graph of one batch --------------------------------------------------
+
+}  // extern "C"
+
+struct fd2_graph {
+  fd2_ctx* c = nullptr;
+  int64_t n = 0;
+  uint32_t* d_chunk_off = nullptr;
+  unsigned long long* d_photon_off = nullptr;
+  uint32_t* d_block_chunks = nullptr;
+  unsigned long long* d_block_photons = nullptr;
+  uint32_t* d_base = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+void fd2_graph_destroy(fd2_graph* g) {
+  if (!g) return;
+  if (g->c) {
+    (void)hipSetDevice(g->c->device);
+    (void)hipStreamSynchronize(g->c->stream);
+  }
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  void* bufs[] = {g->d_chunk_off, g->d_photon_off, g->d_block_chunks, g->d_block_photons,
+                  g->d_base};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete g;
+}
+
+int fd2_graph_create(fd2_ctx* c, const fd2_event* d_events, int64_t n, fd2_graph** out) {
+  if (!c || !out || n <= 0 || !d_events || n >= (int64_t)UINT32_MAX) return FD2_E_INVAL;
+  *out = nullptr;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  fd2_graph* g = new (std::nothrow) fd2_graph();
+  if (!g) return FD2_E_NOMEM;
+  g->c = c;
+  g->n = n;
+  const size_t nb = (size_t)((n + kScanTile - 1) / kScanTile) + 1;
+  int rc = FD2_OK;
+  if (hipMalloc(&g->d_chunk_off, (size_t)(n + 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&g->d_photon_off, (size_t)(n + 1) * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&g->d_block_chunks, nb * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&g->d_block_photons, nb * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&g->d_base, sizeof(uint32_t)) != hipSuccess)
+    rc = FD2_E_NOMEM;
+  if (!rc) {
+    const QueueBufs b{g->d_chunk_off, g->d_photon_off, g->d_block_chunks, g->d_block_photons};
+    Launch L = make_launch(c, b, d_events, n, nullptr, nullptr);
+    L.a.event_base_dev = g->d_base;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      rc = FD2_E_DEVICE;
+    } else {
+      const hipError_t e = issue(L, c->stream);
+      const hipError_t e2 = hipStreamEndCapture(c->stream, &g->graph);
+      if (e != hipSuccess || e2 != hipSuccess ||
+          hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = FD2_E_DEVICE;
+    }
+  }
+  if (rc) {
+    (void)hipGetLastError();
+    fd2_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return FD2_OK;
+}
+
+int fd2_graph_launch(fd2_graph* g) {
+  if (!g || !g->exec) return FD2_E_INVAL;
+  fd2_ctx* c = g->c;
+  if (c->next_event + g->n > (int64_t)UINT32_MAX) return FD2_E_STATE;
+  FD2_HIP(hipSetDevice(c->device));
+  FD2_HIP(hipEventRecord(c->ev_start, c->stream));
+  FD2_HIP(launch_set_u32(g->d_base, (uint32_t)c->next_event, c->stream));
+  FD2_HIP(hipGraphLaunch(g->exec, c->stream));
+  FD2_HIP(hipEventRecord(c->ev_stop, c->stream));
+  c->timed = true;
+  c->next_event += g->n;
+  return FD2_OK;
+}
+
+// ---- autotuner ------------------------------------------------------------------
+
+int fd2_autotune(fd2_ctx* c, const fd2_event* d_events, int64_t n, int32_t repeats,
+                 fd2_tuning* best, fd2_tune_result* results, int32_t max_results,
+                 int32_t* n_results) {
+  if (!c || !d_events || n <= 0 || (max_results > 0 && !results)) return FD2_E_INVAL;
+  if (repeats < 1) repeats = 1;
+  fd2_ctx* s = nullptr;
+  int rc = fd2_create(c->params, c->n_geoms, c->device, &s);
+  if (rc) return rc;
+  fd2_set_seed(s, c->seed);
+  int32_t count = 0;
+  auto time_one = [&](const fd2_tuning& t, float* ms_out) -> int {
+    int r = fd2_set_tuning(s, &t);
+    if (r) return r;
+    float best_ms = 0.0f;
+    for (int k = 0; k <= repeats; ++k) {  // the first run warms up
+      r = fd2_submit_device(s, d_events, n);
+      if (!r) r = fd2_sync(s);
+      float ms = 0.0f;
+      if (!r) r = fd2_last_kernel_ms(s, &ms);
+      if (r) return r;
+      if (k == 1 || (k > 1 && ms < best_ms)) best_ms = ms;
+    }
+    if (count < max_results) {
+      results[count].tuning = t;
+      results[count].ms = best_ms;
+    }
+    ++count;
+    *ms_out = best_ms;
+    return fd2_reset(s);
+  };
+  // coordinate descent from the context's current shape, two sweeps over
+  // chunk, waves per CU, parked lanes and LDS tile side
+  static const int chunks[] = {16, 32, 64, 128, 256};
+  static const int waves[] = {8, 12, 16};
+  static const int parks[] = {8, 16, 32, 48};
+  static const int tiles[] = {-1, 32, 48, 64, 96};
+  static const int refills[] = {1, 4, 8, 16, 32};
+  fd2_tuning cur = c->tune;
+  float cur_ms = 0.0f;
+  rc = time_one(cur, &cur_ms);
+  for (int sweep = 0; sweep < 2 && !rc; ++sweep) {
+    for (int dim = 0; dim < 5 && !rc; ++dim) {
+      const int* vals = dim == 0 ? chunks : dim == 1 ? waves : dim == 2 ? parks
+                      : dim == 3 ? tiles : refills;
+      const int nv = dim == 1 ? 3 : dim == 2 ? 4 : 5;
+      for (int i = 0; i < nv && !rc; ++i) {
+        fd2_tuning t = cur;
+        int32_t* f = dim == 0 ? &t.chunk : dim == 1 ? &t.waves_per_cu
+                   : dim == 2 ? &t.park_lanes : dim == 3 ? &t.lds_tile : &t.refill_lanes;
+        if (*f == vals[i] || (dim == 3 && *f == 0 && vals[i] == kTileDefault)) continue;
+        *f = vals[i];
+        float ms = 0.0f;
+        rc = time_one(t, &ms);
+        if (!rc && ms < cur_ms) {
+          cur = t;
+          cur_ms = ms;
+        }
+      }
+    }
+  }
+  fd2_destroy(s);
+  if (rc) return rc;
+  if (n_results) *n_results = count < max_results ? count : max_results;
+  if (best) *best = cur;
+  return fd2_set_tuning(c, &cur);
 }
 
 }  // extern "C"

@@ -14,18 +14,30 @@ struct TransportArgs {
   const fd2_event* events;             // device
   const uint32_t* chunk_off;           // n_events + 1 (exclusive scan)
   const unsigned long long* photon_off;  // n_events + 1 (exclusive scan)
-  const GeomDev* geoms;                // n_geoms
+  const GeomDev* geoms;                // n_geoms, null prf/hist: tallied by the kernel
+  const GeomDev* tally;                // n_geoms, with the prf/hist pointers
   unsigned long long* counters;        // n_geoms * FD2_NCOUNT
   uint32_t* cursor;                    // work-queue cursor (zeroed by the scan)
   fd2_photon* records;                 // per-photon records (recording only)
   uint32_t* radial;                    // n_events x FD2_RADIAL_BINS (radial mode)
+  const uint32_t* event_base_dev;      // if set, read the number of events[0] here
   int64_t n_events;
   uint32_t seed;
   uint32_t event_base;                 // global number of events[0]
   int32_t n_geoms;
   int32_t chunk_log2;
   int32_t park_lanes;                  // lanes waiting before a Fresnel phase
+  int32_t refill_lanes;                // free lanes before new photons start
+  int32_t tile_side;                   // LDS PRF tile side (pixels), 0 = none
+  int32_t n_tiles;                     // geometries [0, n_tiles) have a tile
 };
+
+// LDS PRF tiles of the persistent kernel: geometry g < n_tiles counts the
+// detections in a tile_side^2 pixel square centred on its PRF image in LDS
+// (u32) and adds the block's counts to the PRF at the end.
+constexpr int kTileDefault = 96;
+constexpr int kTileBudget = 96 * 96;   // u32 entries per block (36 KiB)
+constexpr int kTileMax = 120;  // 57.6 KiB: within 64 KiB of LDS per workgroup
 
 struct QueueArgs {
   const fd2_event* events;
@@ -63,7 +75,8 @@ hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
                               hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
-int transport_max_blocks_per_cu(bool multi, bool record, bool radial);
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem);
 
 }  // namespace fd2
 
@@ -78,7 +91,8 @@ struct fd2_ctx {
   int n_geoms = 0;
   fd2_params params[FD2_MAX_GEOMS];
   fd2::GeomDev hgeom[FD2_MAX_GEOMS];
-  fd2::GeomDev* d_geoms = nullptr;
+  fd2::GeomDev* d_geoms = nullptr;     // with the tally pointers
+  fd2::GeomDev* d_geoms_mc = nullptr;  // null tally pointers (persistent kernel)
   unsigned long long* d_prf[FD2_MAX_GEOMS] = {};
   uint32_t* d_hist[FD2_MAX_GEOMS] = {};
   unsigned long long* d_counters = nullptr;

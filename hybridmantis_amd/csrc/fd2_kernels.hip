@@ -234,12 +234,28 @@ __device__ __forceinline__ int radial_bin(float dx, float dy) {
   return b < 1 ? 1 : (b > FD2_RADIAL_BINS - 2 ? FD2_RADIAL_BINS - 2 : b);
 }
 
+// first pixel row/column of a geometry's LDS tile
+__device__ __forceinline__ int tile_origin(int dim, int side) {
+  return dim > side ? (dim - side) >> 1 : 0;
+}
+
 template <bool MULTI, bool REC, bool RADIAL>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
 k_transport(TransportArgs a) {
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
+  extern __shared__ uint32_t s_tile[];  // n_tiles x tile_side^2
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x) s_cnt[i] = 0;
+  const int side = a.tile_side;
+  const int tile_area = side * side;
+  for (int i = threadIdx.x; i < a.n_tiles * tile_area; i += blockDim.x) s_tile[i] = 0;
   __syncthreads();
+
+  // per-lane run of detections of one history, added to the history's
+  // count when the lane moves on to another history (photons of a chunk
+  // share their event)
+  int32_t run_hist = 0;
+  uint32_t run_g = 0, run_cnt = 0;
+  const uint32_t event_base = a.event_base_dev ? *a.event_base_dev : a.event_base;
 
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
@@ -260,7 +276,11 @@ k_transport(TransportArgs a) {
   StepSink nosink{nullptr, nullptr, 0, 0, false};
 
   for (;;) {
+    // Starting a photon (stream seeding, first direction) is long and would
+    // run with one or two lanes if done whenever a photon ends: wait until
+    // refill_lanes lanes are free, or no lane is left to step.
     unsigned long long need = __ballot(!alive);
+    if (__popcll(need) < a.refill_lanes && need != ~0ull) need = 0ull;
     while (need != 0ull && !pool_done) {
       if (pool_next >= pool_end) {
         uint32_t c = 0;
@@ -293,7 +313,7 @@ k_transport(TransportArgs a) {
         const GeomDev& g = a.geoms[gi];
         const uint32_t k = (uint32_t)(pool_next + rank);
         start_photon(p, ev);
-        rng.seed(a.seed, a.event_base + (uint32_t)pool_ev, k);
+        rng.seed(a.seed, event_base + (uint32_t)pool_ev, k);
         counter_initial_dir(rng, p, k, g);
         if (REC) rec_idx = a.photon_off[pool_ev] + k;
         if (RADIAL) lane_ev = pool_ev;
@@ -329,6 +349,32 @@ k_transport(TransportArgs a) {
     }
     if (ended) {
       atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
+      if (p.fate == kDetect || p.fate == kDetectOut) {
+        const GeomDev& gt = a.tally[MULTI ? gi : 0u];
+        if (p.fate == kDetect) {
+          bool in_tile = false;
+          if ((int)gi < a.n_tiles) {
+            const int i = p.pix / gt.ydim;
+            const uint32_t ti = (uint32_t)(i - tile_origin(gt.xdim, side));
+            const uint32_t tj = (uint32_t)(p.pix - i * gt.ydim - tile_origin(gt.ydim, side));
+            if (ti < (uint32_t)side && tj < (uint32_t)side) {
+              atomicAdd(&s_tile[gi * tile_area + ti * side + tj], 1u);
+              in_tile = true;
+            }
+          }
+          if (!in_tile) atomicAdd(&gt.prf[p.pix], 1ull);
+        }
+        if (p.hist >= 1 && p.hist <= gt.num_hist) {
+          if (p.hist == run_hist && (!MULTI || gi == run_g)) {
+            ++run_cnt;
+          } else {
+            if (run_cnt) atomicAdd(&a.tally[MULTI ? run_g : 0u].hist[run_hist - 1], run_cnt);
+            run_hist = p.hist;
+            run_g = gi;
+            run_cnt = 1;
+          }
+        }
+      }
       if (RADIAL && (p.fate == kDetect || p.fate == kDetectOut))
         atomicAdd(&a.radial[lane_ev * FD2_RADIAL_BINS + radial_bin(p.fx - p.ex, p.fy - p.ey)], 1u);
       if (REC) {
@@ -343,14 +389,29 @@ k_transport(TransportArgs a) {
     }
   }
 
+  if (run_cnt) atomicAdd(&a.tally[MULTI ? run_g : 0u].hist[run_hist - 1], run_cnt);
   __syncthreads();
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x)
     if (s_cnt[i]) atomicAdd(&a.counters[i], (unsigned long long)s_cnt[i]);
+  for (int t = 0; t < a.n_tiles; ++t) {
+    const GeomDev& gt = a.tally[t];
+    const int i0 = tile_origin(gt.xdim, side), j0 = tile_origin(gt.ydim, side);
+    for (int e = threadIdx.x; e < tile_area; e += blockDim.x) {
+      const uint32_t v = s_tile[t * tile_area + e];
+      if (v) {
+        const int ti = e / side;
+        atomicAdd(&gt.prf[(size_t)(i0 + ti) * gt.ydim + (j0 + e - ti * side)],
+                  (unsigned long long)v);
+      }
+    }
+  }
 }
 
 template <bool MULTI, bool REC, bool RADIAL>
 static void launch_one(const TransportArgs& a, dim3 grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL>), grid, dim3(kTransportThreads), 0, s, a);
+  const size_t smem = (size_t)a.n_tiles * a.tile_side * a.tile_side * sizeof(uint32_t);
+  hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL>), grid, dim3(kTransportThreads), smem, s,
+                     a);
 }
 
 hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
@@ -370,18 +431,20 @@ hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
 }
 
 template <bool MULTI, bool REC, bool RADIAL>
-static int occupancy_of() {
+static int occupancy_of(size_t smem) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<MULTI, REC, RADIAL>,
-                                                   kTransportThreads, 0) != hipSuccess)
+                                                   kTransportThreads, smem) != hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;
 }
 
-int transport_max_blocks_per_cu(bool multi, bool record, bool radial) {
-  if (radial) return multi ? occupancy_of<true, false, true>() : occupancy_of<false, false, true>();
-  if (multi) return record ? occupancy_of<true, true, false>() : occupancy_of<true, false, false>();
-  return record ? occupancy_of<false, true, false>() : occupancy_of<false, false, false>();
+int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem) {
+  if (radial)
+    return multi ? occupancy_of<true, false, true>(smem) : occupancy_of<false, false, true>(smem);
+  if (multi)
+    return record ? occupancy_of<true, true, false>(smem) : occupancy_of<true, false, false>(smem);
+  return record ? occupancy_of<false, true, false>(smem) : occupancy_of<false, false, false>(smem);
 }
 
 // ------------------------------------------------ RANECU trajectory mode
@@ -511,6 +574,15 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const size_t nb = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0) p[0] = v;
+}
+
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, p, v);
   return hipGetLastError();
 }
 

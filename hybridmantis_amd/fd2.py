@@ -48,7 +48,8 @@ class Params(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("waves_per_cu", C.c_int32),
-                ("lds_tile", C.c_int32), ("park_lanes", C.c_int32)]
+                ("lds_tile", C.c_int32), ("park_lanes", C.c_int32),
+                ("refill_lanes", C.c_int32)]
 
 
 # dict keys (events.BASE_PARAMS) -> fd2_params members
@@ -108,6 +109,11 @@ def lib() -> C.CDLL:
         "fd2_device_free": ([P, P], C.c_int),
         "fd2_copy_to_device": ([P, P, P, C.c_size_t], C.c_int),
         "fd2_strerror": ([C.c_int], C.c_char_p),
+        "fd2_graph_create": ([P, P, C.c_int64, C.POINTER(P)], C.c_int),
+        "fd2_graph_launch": ([P], C.c_int),
+        "fd2_graph_destroy": ([P], None),
+        "fd2_autotune": ([P, P, C.c_int64, C.c_int32, P, P, C.c_int32,
+                          C.POINTER(C.c_int32)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -165,6 +171,35 @@ class DeviceBuffer:
             self.ptr = None
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("tuning", Tuning), ("ms", C.c_float)]
+
+
+class Graph:
+    """fd2_graph: a captured batch, replayed by launch()."""
+
+    def __init__(self, tr: "Transport", buf: DeviceBuffer, n: int):
+        self._tr = tr
+        self.n = n
+        h = C.c_void_p()
+        _check(lib().fd2_graph_create(tr._ctx, buf.ptr, n, C.byref(h)), "fd2_graph_create")
+        self._h = h
+
+    def launch(self) -> None:
+        _check(lib().fd2_graph_launch(self._h), "fd2_graph_launch")
+
+    def close(self) -> None:
+        if self._h:
+            lib().fd2_graph_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class Transport:
     """One GPU context: parameter sets (geometries), tallies, work queue."""
 
@@ -200,15 +235,15 @@ class Transport:
 
     # -- configuration
     def set_tuning(self, chunk: int = 0, waves_per_cu: int = 0, lds_tile: int = 0,
-                   park_lanes: int = 0):
-        t = Tuning(chunk, waves_per_cu, lds_tile, park_lanes)
+                   park_lanes: int = 0, refill_lanes: int = 0):
+        t = Tuning(chunk, waves_per_cu, lds_tile, park_lanes, refill_lanes)
         _check(lib().fd2_set_tuning(self._ctx, C.byref(t)), "set_tuning")
 
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().fd2_get_tuning(self._ctx, C.byref(t)), "get_tuning")
         return dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu, lds_tile=t.lds_tile,
-                    park_lanes=t.park_lanes)
+                    park_lanes=t.park_lanes, refill_lanes=t.refill_lanes)
 
     def set_seed(self, seed: int) -> None:
         _check(lib().fd2_set_seed(self._ctx, seed & 0xFFFFFFFF), "set_seed")
@@ -252,6 +287,33 @@ class Transport:
 
     def device_buffer(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
+
+    def upload_events(self, events) -> DeviceBuffer:
+        """Events copied to a new device buffer (for submit_device, graph,
+        autotune)."""
+        ev = as_events(events)
+        buf = DeviceBuffer(self, max(ev.nbytes, 1))
+        buf.upload(ev)
+        buf.n = len(ev)
+        return buf
+
+    def graph(self, buf: DeviceBuffer, n: int) -> "Graph":
+        """HIP graph of the queue build + transport of n device events."""
+        return Graph(self, buf, n)
+
+    def autotune(self, buf: DeviceBuffer, n: int, repeats: int = 2,
+                 max_results: int = 64):
+        """Pick the fastest launch shape on these events (fd2_autotune); it
+        is set on this context.  Returns (best tuning, [(tuning, ms), ...])."""
+        best = Tuning()
+        res = (TuneResult * max_results)()
+        nres = C.c_int32(0)
+        _check(lib().fd2_autotune(self._ctx, buf.ptr, n, repeats, C.byref(best), res,
+                                  max_results, C.byref(nres)), "fd2_autotune")
+        as_dict = lambda t: dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu,  # noqa: E731
+                                 lds_tile=t.lds_tile, park_lanes=t.park_lanes,
+                                 refill_lanes=t.refill_lanes)
+        return as_dict(best), [(as_dict(r.tuning), float(r.ms)) for r in res[:nres.value]]
 
     # -- tallies
     def prf_dims(self, geom: int = 0):

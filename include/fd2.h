@@ -106,9 +106,14 @@ enum {
 typedef struct fd2_tuning {
   int32_t chunk;            /* photons per work-queue item (power of 2)     */
   int32_t waves_per_cu;     /* resident waves per CU (workgroups of 4 waves) */
-  int32_t lds_tile;         /* side of the LDS PRF tile (0 = off)           */
+  int32_t lds_tile;         /* side (pixels, <= 120) of the LDS PRF tiles,
+                             * centred on each PRF image, of the first
+                             * min(n_geoms, 9216 / side^2) geometries;
+                             * 0 = default (96), -1 = no tiles              */
   int32_t park_lanes;       /* lanes of a wave that wait for the Fresnel /
                              * inter-column phase before it runs (1-64)     */
+  int32_t refill_lanes;     /* free lanes of a wave before new photons are
+                             * started (1-64)                               */
 } fd2_tuning;
 
 typedef struct fd2_ctx fd2_ctx;
@@ -258,6 +263,38 @@ typedef struct fd2_surrogate_stats {
 int fd2_surrogate_submit(fd2_ctx *ctx, fd2_mlp *mlp, const fd2_event *events,
                          int64_t n, const fd2_surrogate_cfg *cfg,
                          fd2_surrogate_stats *stats);
+
+/* ---- batches as HIP graphs ----------------------------------------------- */
+
+/* Capture the work-queue build and the persistent transport of n events in
+ * device memory (valid for the graph's life) into a HIP graph with its own
+ * queue buffers.  Each fd2_graph_launch replays it asynchronously on the
+ * context's stream (fd2_sync waits): the events get the context's next n
+ * event numbers, as an fd2_submit_device of them would, so every replay is
+ * a new batch with its own random streams, tallied into the context.  The
+ * launch shape is the context's tuning at capture.  No counterpart in the
+ * reference (one kernel launch per buffer, drv-G:277). */
+typedef struct fd2_graph fd2_graph;
+int fd2_graph_create(fd2_ctx *ctx, const fd2_event *d_events, int64_t n,
+                     fd2_graph **out);
+int fd2_graph_launch(fd2_graph *graph);
+void fd2_graph_destroy(fd2_graph *graph);
+
+/* Launch-shape autotuner: times the transport of n device events on a
+ * scratch context with the same parameters (the context's tallies are not
+ * touched) for candidate shapes -- coordinate descent, two sweeps over chunk
+ * {16..256}, waves per CU {8, 12, 16}, parked lanes {8..48}, LDS tile side
+ * {none, 32..96} and refill lanes {1..32}, each the best of `repeats` runs
+ * after a warm-up --
+ * and sets the fastest on ctx.  Candidates and their times go to results
+ * (up to max_results; *n_results = number written). */
+typedef struct fd2_tune_result {
+  fd2_tuning tuning;
+  float ms;
+} fd2_tune_result;
+int fd2_autotune(fd2_ctx *ctx, const fd2_event *d_events, int64_t n, int32_t repeats,
+                 fd2_tuning *best, fd2_tune_result *results, int32_t max_results,
+                 int32_t *n_results);
 
 /* ---- reference file formats -------------------------------------------- */
 

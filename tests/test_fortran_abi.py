@@ -100,3 +100,23 @@ def test_poisson_streams_are_keyed_by_ordinal():
         F.poisson(np.array([-1.0]), 5)
     with pytest.raises(Exception):
         F.poisson(np.array([np.nan]), 5)
+
+
+def test_entry_point_errors_abort_with_a_message():
+    """Errors in the entry points print a message and abort (the reference's
+    cutilSafeCall behaviour); no silent return.  Here: a PHS of 2000 bins
+    does not fit /outputdetprim/, or (without a GPU) no device."""
+    import signal
+    import sys
+
+    code = (
+        "from hybridmantis_amd import events as E, fortran as F\n"
+        "C = F.commons()\n"
+        "C.set_params(E.params(num_bins=2000))\n"
+        "F.gpuoptical(99, 10)\n"
+        "print('returned')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == -signal.SIGABRT, (r.returncode, r.stdout, r.stderr)
+    assert "returned" not in r.stdout
+    assert "fastDETECT2 (gpuoptical)" in r.stderr

@@ -21,7 +21,7 @@ from hybridmantis_amd import fortran as F
 pytestmark = pytest.mark.gpu
 
 SEED = 4242
-COUNTERS = [("glgen", "generated"), ("gldetect", "detect"), ("glabstop", "abs_top"),
+COUNTERS = [("glgen", "generated"), ("gldetect", "detected"), ("glabstop", "abs_top"),
             ("glabsbulk", "abs_bulk"), ("gllost", "lost"), ("gloutofcol", "outofcol"),
             ("gltheta1", "theta1")]
 
@@ -32,6 +32,14 @@ def shim():
     C = F.commons()
     yield C
     os.environ.pop("FD2_SEED", None)
+
+
+def shim_params(p):
+    """The COMMON /outputdetprim/ block holds 1000 PHS bins (the entry
+    points abort on more): 1000 bins of 2 photons."""
+    q = dict(p, num_bins=1000, max_ph=2000, min_ph=0)
+    assert q["num_bins"] <= 1000 and (q["ubx"] - q["lbx"]) / q["pixel"] <= 501
+    return q
 
 
 def as_buffer_events(ev, p, seed, first):
@@ -61,6 +69,7 @@ def image_part(C, prf):
 def test_gpuoptical_protocol(shim):
     C = shim
     p, ev = E.workload("mono25", scale=0.25)     # 500 histories
+    p = shim_params(p)
     C.set_params(p)
     C.clear_outputs()
     cuts = [0, len(ev) // 3, 2 * len(ev) // 3, len(ev)]
@@ -94,6 +103,7 @@ def test_cpuoptical_per_call_phs_and_range(shim):
     that call's events only."""
     C = shim
     p, ev = E.workload("mono25", scale=0.1)
+    p = shim_params(p)
     C.set_params(p)
     C.clear_outputs()
     C.fill(ev)
@@ -119,6 +129,7 @@ def test_load_balancing_probe(shim):
     cpuoptlb_ leaves every output untouched and reports +inf."""
     C = shim
     p, ev = E.workload("mono25", scale=0.05)
+    p = shim_params(p)
     C.set_params(p)
     C.clear_outputs()
     C.fill(ev)
