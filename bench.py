@@ -125,7 +125,7 @@ def surrogate_leg() -> dict:
 
     r = surrogate_eval.evaluate(1.0)
     keep = ("histories", "events", "photons", "mc_histories_per_s", "surrogate_histories_per_s",
-            "speedup", "prf_l1", "detected_fraction_rel_err", "swank_rel_err",
+            "speedup", "prf_l1", "prf_l1_9um", "detected_fraction_rel_err", "swank_rel_err",
             "mean_detected_rel_err", "mlp_forward_tflops", "mlp_forward_mfma_frac")
     out = {k: r[k] for k in keep if k in r}
     rt = r["routing"]

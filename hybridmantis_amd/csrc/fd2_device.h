@@ -47,6 +47,9 @@ struct GeomDev {
   float dspan;         // d_max - d_min
   float eps_lt, eps_gt;    // x < eps, x > eps
   float norm_lo, norm_hi;  // |v| < 1 - eps, |v| > 1 + eps
+  // bulk absorption shortcut: a step of length <= abs_ds_cap with uniform
+  // u < abs_u_cap is certainly shorter than -ln(u)/bulk (1% margin)
+  float abs_ds_cap, abs_u_cap;
   int32_t pixel, xdim, ydim, num_hist;
   int32_t gap_fresnel; // (n1/n2) >= 1.57: gap side recomputes Pr/Pt
   int32_t pad0;
@@ -140,7 +143,8 @@ __device__ __forceinline__ void ranecu_init(RanecuRng& g, int batch,
 
 enum Fate : int {
   kNone = 0, kDetect = 1, kDetectOut = 2, kAbsTop = 3, kAbsBulk = 4,
-  kLost = 5, kOutOfCol = 6, kTheta1 = 7
+  kLost = 5, kOutOfCol = 6, kTheta1 = 7,
+  kResume = 8  // alive: reflected by the sensor, its next in-column step is due
 };
 
 // photon state, all in registers
@@ -415,7 +419,15 @@ __device__ __forceinline__ bool isotropic(Rng& rng, Photon& p, const GeomDev& g,
                                           StepSink* sink) {
   const float ds = dist_to_surface<TRACE>(p, g, sink);
   float da = 999999.0f;
-  if (g.bulk > 0.0f) da = g.neg_inv_bulk * fd2_log(rng.u());
+  if (g.bulk > 0.0f) {
+    const float u = rng.u();
+    // the logarithm only decides the outcome when u is close to 1 or the
+    // step is long; otherwise da > ds is certain and the outcome the same
+    if (ds >= 0.0f && ds <= g.abs_ds_cap && u < g.abs_u_cap)
+      da = 3.0e38f;
+    else
+      da = g.neg_inv_bulk * fd2_log(u);
+  }
   if (fabsf(ds - (-99.0f)) < g.eps_lt) return true;
   if (ds < da && ds >= 0.0f) return false;
   if (ds >= da && ds >= 0.0f) {
@@ -426,23 +438,27 @@ __device__ __forceinline__ bool isotropic(Rng& rng, Photon& p, const GeomDev& g,
 }
 
 // detection (main/kernel_cuda_c_ver1_0.cu:3372-3427): on the sensor plane,
-// specular reflection with probability sensorRefl (then another in-column
-// step), else detected.  true = photon ended.
-template <class Rng, bool TRACE>
-__device__ __forceinline__ bool detection(Rng& rng, Photon& p, const GeomDev& g,
-                                          StepSink* sink) {
+// specular reflection with probability sensorRefl, else detected.  The
+// reference follows the reflection with another in-column step inside the
+// same pass; here the pass returns kSensorReflect and the caller runs that
+// step as the start of the next pass (hot_step), so the step's code exists
+// once.
+enum SensorResult : int { kSensorNone = 0, kSensorDetect = 1, kSensorReflect = 2 };
+
+template <class Rng>
+__device__ __forceinline__ int detection(Rng& rng, Photon& p, const GeomDev& g) {
   if (fabsf(p.pz - (-g.half_h)) < g.eps_lt) {
     if (rng.u() < g.sensor_refl) {
       p.nx = 0.0f;
       p.ny = 0.0f;
       p.nz = 1.0f;
       reflect(p, g);
-      return isotropic<Rng, TRACE>(rng, p, g, sink);
+      return kSensorReflect;
     }
     tally_detect(p, g);
-    return true;
+    return kSensorDetect;
   }
-  return false;
+  return kSensorNone;
 }
 
 // New column orientation (transmit/boundary_analysis/refl_bottom,
@@ -822,12 +838,24 @@ enum StepResult : int { kAlive = 0, kEnded = 1, kTrapped = 2, kNeedPrpt = 3 };
 template <class Rng, bool TRACE>
 __device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
                                         StepSink* sink) {
-  if (++p.loops > kMaxLoops) {
+  // a pass resumed after a sensor reflection is the rest of the previous
+  // pass: no loop count, no second sensor check
+  const bool resumed = p.fate == kResume;
+  if (resumed) {
+    p.fate = kNone;
+  } else if (++p.loops > kMaxLoops) {
     p.fate = kLost;
     return kEnded;
   }
   if (isotropic<Rng, TRACE>(rng, p, g, sink)) return kEnded;
-  if (detection<Rng, TRACE>(rng, p, g, sink)) return kEnded;
+  if (!resumed) {
+    const int d = detection(rng, p, g);
+    if (d == kSensorDetect) return kEnded;
+    if (d == kSensorReflect) {
+      p.fate = kResume;
+      return kAlive;
+    }
+  }
   if (fabsf(p.uz) < g.eps_lt) return kTrapped;
   p.first = false;
   const int b = ba_entry(rng, p, g);

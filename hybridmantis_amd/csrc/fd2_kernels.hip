@@ -228,9 +228,9 @@ __device__ __forceinline__ void counter_initial_dir(CounterRng& rng, Photon& p,
 __device__ __forceinline__ int radial_bin(float dx, float dy) {
   const float r = sqrtf(dx * dx + dy * dy);
   if (!(r >= 0.5f)) return 0;
-  if (r >= 200.0f) return FD2_RADIAL_BINS - 1;
-  // 1 + floor(62 * ln(r/0.5) / ln(400))
-  int b = 1 + (int)(fd2_log(r * 2.0f) * 10.3479490f);
+  if (r >= 800.0f) return FD2_RADIAL_BINS - 1;
+  // 1 + floor(62 * ln(r/0.5) / ln(1600))
+  int b = 1 + (int)(fd2_log(r * 2.0f) * 8.40363595f);
   return b < 1 ? 1 : (b > FD2_RADIAL_BINS - 2 ? FD2_RADIAL_BINS - 2 : b);
 }
 

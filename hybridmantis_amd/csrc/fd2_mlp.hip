@@ -704,8 +704,8 @@ __global__ void k_surr_features(const fd2_event* ev, int n, const fd2::GeomDev* 
 // edges of the radial bins: 0, 0.5 * 400^((j-1)/62) for j = 1..63, 300 um
 __device__ __forceinline__ float radial_edge(int j) {
   if (j <= 0) return 0.0f;
-  if (j >= FD2_RADIAL_BINS) return 300.0f;
-  return 0.5f * expf((float)(j - 1) * 0.0966378962f);  // ln(400)/62
+  if (j >= FD2_RADIAL_BINS) return 1000.0f;
+  return 0.5f * expf((float)(j - 1) * 0.118996111f);  // ln(1600)/62
 }
 
 struct SampleArgs {
@@ -837,6 +837,8 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
     cfg.h_min = 100.0f; cfg.h_max = 500.0f;
     cfg.r_min = 2.55f; cfg.r_max = 5.0f;
     cfg.refl_min = 0.5f; cfg.refl_max = 0.95f;
+    cfg.center_band = 25.0f;
+    cfg.det_extent = 1001.0f;
   }
   fd2_surrogate_stats stats;
   std::memset(&stats, 0, sizeof stats);
@@ -851,7 +853,15 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
   for (int g = 0; g < c->n_geoms; ++g) {
     const fd2_params& q = c->params[g];
     const double refl = 1.0 - q.dettop;
-    in_range[g] = q.detheight >= cfg.h_min && q.detheight <= cfg.h_max &&
+    // parameters the network does not take as input must be the training
+    // set's (hybridmantis_amd/events.py BASE_PARAMS)
+    const bool fixed = std::fabs(q.detnC - 1.8) < 1e-9 && std::fabs(q.detnIC - 1.0) < 1e-9 &&
+                       std::fabs(q.detbulk - 1e-4) < 1e-12 && std::fabs(q.detbeta - 0.2) < 1e-9 &&
+                       std::fabs(q.detdmin - 1.0) < 1e-9 && std::fabs(q.detdmax - 280.0) < 1e-9 &&
+                       q.detsensorRefl == 0.0 &&
+                       std::fabs(q.detx - cfg.det_extent) < 1e-3 &&
+                       std::fabs(q.dety - cfg.det_extent) < 1e-3;
+    in_range[g] = fixed && q.detheight >= cfg.h_min && q.detheight <= cfg.h_max &&
                   q.detradius >= cfg.r_min && q.detradius <= cfg.r_max &&
                   refl >= cfg.refl_min && refl <= cfg.refl_max;
   }
@@ -863,7 +873,10 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
     const fd2::GeomDev& g = c->hgeom[e.geom];
     const uint32_t gid = (uint32_t)(c->next_event + i);
     const bool near = e.z > g.half_h - cfg.surface_band || e.z < -g.half_h + cfg.surface_band;
-    if (near || !in_range[e.geom]) {
+    const float ox = e.x - 0.5f * g.xdet, oy = e.y - 0.5f * g.ydet;
+    const bool off_centre = ox * ox + oy * oy > cfg.center_band * cfg.center_band;
+    const bool off_axis = !(e.col_dx >= 0.0f && e.col_dx < g.R);  // trained on [0, R)
+    if (near || off_centre || off_axis || !in_range[e.geom]) {
       mc.push_back(e);
       stats.photons_mc += e.N;
     } else if ((uint64_t)fd2mlp::mix32(gid ^ fd2mlp::mix32(cfg.seed)) < audit_cut) {

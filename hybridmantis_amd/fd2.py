@@ -455,7 +455,8 @@ class SurrogateCfg(C.Structure):
                 ("h_min", C.c_float), ("h_max", C.c_float),
                 ("r_min", C.c_float), ("r_max", C.c_float),
                 ("refl_min", C.c_float), ("refl_max", C.c_float),
-                ("seed", C.c_uint32), ("reserved", C.c_int32)]
+                ("seed", C.c_uint32), ("center_band", C.c_float), ("det_extent", C.c_float),
+                ("reserved", C.c_int32)]
 
 
 class SurrogateStats(C.Structure):
@@ -467,7 +468,8 @@ class SurrogateStats(C.Structure):
 
 
 SURROGATE_DEFAULTS = dict(surface_band=5.0, audit_fraction=0.01, h_min=100.0, h_max=500.0,
-                          r_min=2.55, r_max=5.0, refl_min=0.5, refl_max=0.95, seed=0)
+                          r_min=2.55, r_max=5.0, refl_min=0.5, refl_max=0.95, seed=0,
+                          center_band=25.0, det_extent=1001.0)
 
 
 def surrogate_submit(tr: "Transport", net: "Mlp", events, **cfg) -> dict:

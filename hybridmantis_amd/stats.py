@@ -67,8 +67,8 @@ def radial_profile(prf, center_px, edges_px) -> np.ndarray:
     return out
 
 
-def log_radial_bins(r_min: float = 0.5, r_max: float = 200.0, n: int = 64) -> np.ndarray:
-    """Edges of the 64-bin log-radial PRF (0.5-200 um) plus an inner bin."""
+def log_radial_bins(r_min: float = 0.5, r_max: float = 800.0, n: int = 64) -> np.ndarray:
+    """Edges of a log-radial profile (default 0.5-800 um) plus an inner bin."""
     return np.concatenate([[0.0], np.geomspace(r_min, r_max, n)])
 
 

@@ -4,8 +4,8 @@ For a photon emitted at depth d below the top surface, offset col_dx from
 its first column's axis, in a layer of thickness H with column radius R and
 top reflectivity rho, the network predicts the probability that the photon
 is detected and the distribution of its detection point's distance from the
-emission point in 64 log-radial bins (0.5-200 um, plus [0, 0.5) and
-[200, 300) um).  Training targets come from the plain Monte Carlo
+emission point in 64 log-radial bins (0.5-800 um, plus [0, 0.5) and
+[800, 1000) um).  Training targets come from the plain Monte Carlo
 (fd2_submit_radial): each sample is one event of many photons at a grid
 point of (d, col_dx, H, R, rho).
 
@@ -29,7 +29,7 @@ TRAIN_REFL = np.linspace(0.5, 0.95, 5)            # top reflectivities
 
 def radial_edges() -> np.ndarray:
     """65 bin edges (um) of the network's radial distribution."""
-    return np.concatenate([[0.0], 0.5 * 400.0 ** (np.arange(63) / 62.0), [300.0]])
+    return np.concatenate([[0.0], 0.5 * 1600.0 ** (np.arange(63) / 62.0), [1000.0]])
 
 
 def grid_params(geoms=TRAIN_GEOMS, refls=TRAIN_REFL):

@@ -11,8 +11,10 @@ from the network.  Both start from the events in host memory and are timed
 to the tallies being final (wall clock, so routing, copies and launches are
 included).  Reported: histories/s of both and their ratio, the normalised
 PRF L1 distance, the detected-fraction error, and mean detected photons and
-Swank factor of both PHS.  With --noise-floor a second plain-MC run with
-another seed gives the L1 distance two MC runs of this size have anyway.
+Swank factor of both PHS.  The PRF L1 distance is given at the workload's
+1-um pitch and at 9 um (the reference example's pitch).  With --noise-floor
+a second plain-MC run with another seed gives the L1 distances two MC runs
+of this size have anyway.
 """
 from __future__ import annotations
 
@@ -71,6 +73,10 @@ def evaluate(scale: float = 1.0, net=None, seed: int = 271828182, noise_floor: b
         det_mc, det_s = c_mc["detected"] / gen, c_s["detected"] / float(c_s["generated"])
         sw_mc, sw_s = stats.swank(hist_mc), stats.swank(hist_s)
         md_mc, md_s = stats.mean_detected(hist_mc), stats.mean_detected(hist_s)
+        def coarse(img, k=9):  # the reference example's 9-um pitch
+            n = (img.shape[0] // k) * k
+            return img[:n, :n].reshape(n // k, k, n // k, k).sum((1, 3))
+
         out = dict(
             workload="mono50", histories=n_hist, events=int(len(ev)), photons=int(gen),
             mc_seconds=t_mc, mc_kernel_ms=k_ms, mc_histories_per_s=n_hist / t_mc,
@@ -78,6 +84,7 @@ def evaluate(scale: float = 1.0, net=None, seed: int = 271828182, noise_floor: b
             surrogate_seconds=t_s, surrogate_histories_per_s=n_hist / t_s,
             speedup=t_mc / t_s,
             prf_l1=stats.l1_distance(prf_s, prf_mc),
+            prf_l1_9um=stats.l1_distance(coarse(prf_s), coarse(prf_mc)),
             detected_fraction_mc=det_mc, detected_fraction_surrogate=det_s,
             detected_fraction_rel_err=abs(det_s / det_mc - 1.0),
             mean_detected_mc=md_mc, mean_detected_surrogate=md_s,
@@ -92,6 +99,7 @@ def evaluate(scale: float = 1.0, net=None, seed: int = 271828182, noise_floor: b
         if noise_floor:
             _, prf_mc2, hist_mc2, c_mc2, _ = _mc(p, ev, seed + 2)
             out["prf_l1_mc_vs_mc"] = stats.l1_distance(prf_mc2, prf_mc)
+            out["prf_l1_9um_mc_vs_mc"] = stats.l1_distance(coarse(prf_mc2), coarse(prf_mc))
             out["detected_fraction_mc_vs_mc"] = abs(c_mc2["detected"] / c_mc["detected"] - 1)
             out["swank_mc_vs_mc"] = abs(stats.swank(hist_mc2) / sw_mc - 1)
         return out

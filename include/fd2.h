@@ -37,7 +37,8 @@ extern "C" {
 #define FD2_MAX_PRF_SIDE 4096
 #define FD2_MAX_GEOMS 32
 /* log-radial tally of detection offsets from the event: bin 0 = [0, 0.5) um,
- * bins 1..62 log-spaced over [0.5, 200) um, bin 63 = [200, inf) um */
+ * bins 1..62 log-spaced over [0.5, 800) um, bin 63 = [800, inf) um (the
+ * surrogate samples bin 63 over [800, 1000) um) */
 #define FD2_RADIAL_BINS 64
 
 /* Detector and tally parameters: the reference's COMMON /inputargs/ block,
@@ -236,8 +237,14 @@ int fd2_mlp_train(fd2_mlp *mlp, int steps, int batch, float lr, uint32_t seed,
 
 /* Surrogate transport with plain-MC fallback.  Each event goes to the plain
  * Monte Carlo when it lies within surface_band of the top or bottom surface,
- * when its geometry is outside the network's training ranges, or (audit)
- * with probability audit_fraction; all other events are transported by the
+ * farther than center_band from the detector's centre (the network was
+ * trained on events there: photons spread to the detector's sides), when
+ * its geometry is outside the network's training domain -- thickness,
+ * radius and top reflectivity outside the ranges below, a detector extent
+ * other than det_extent, or another value of the parameters the network
+ * does not take as input (n_C 1.8, n_IC 1.0, bulk 1e-4/um, beta 0.2,
+ * d_min 1, d_max 280 um, sensor reflectivity 0) -- or (audit) with
+ * probability audit_fraction.  All other events are transported by the
  * network: per event a binomial number of detected photons and, for each,
  * a radius drawn from the predicted radial distribution and a uniform
  * azimuth, tallied like the Monte Carlo's detections.  Audit events are
@@ -250,6 +257,8 @@ typedef struct fd2_surrogate_cfg {
   float r_min, r_max;       /* column radius                                */
   float refl_min, refl_max; /* and top reflectivity                         */
   uint32_t seed;
+  float center_band;        /* um from the detector centre (default 25)     */
+  float det_extent;         /* detx = dety of the training set (default 1001) */
   int32_t reserved;
 } fd2_surrogate_cfg;
 typedef struct fd2_surrogate_stats {
