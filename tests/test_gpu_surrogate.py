@@ -139,3 +139,24 @@ def test_surrogate_pipeline_small(fd2lib):
     assert int(prf.sum()) <= c["detected"]
     assert abs(c["detected"] / c_mc["detected"] - 1) < 0.1
     assert stats.l1_distance(prf, prf_mc) < 0.5
+
+
+def test_trained_surrogate_gain(fd2lib):
+    """The shipped network (hybridmantis_amd/data/surrogate_mlp.npz) against
+    the method without it: 50,000 histories at 50 keV on a 500-um layer,
+    surrogate + MC fallback vs the plain MC on the same events.  It must be
+    several times faster and agree to the MC's own run-to-run noise: PRF L1
+    within 1.5x the L1 between two plain-MC runs (+0.01), detected fraction
+    and Swank factor within 1%."""
+    from hybridmantis_amd import surrogate_eval
+
+    r = surrogate_eval.evaluate(0.05, noise_floor=True)
+    print({k: r[k] for k in ("speedup", "prf_l1", "prf_l1_mc_vs_mc", "prf_l1_9um",
+                             "prf_l1_9um_mc_vs_mc", "detected_fraction_rel_err",
+                             "swank_rel_err")}, flush=True)
+    assert r["routing"]["n_surrogate"] > 0.9 * r["events"]
+    assert r["speedup"] > 5.0
+    assert r["detected_fraction_rel_err"] < 0.01
+    assert r["swank_rel_err"] < 0.01
+    assert r["prf_l1"] < 1.5 * r["prf_l1_mc_vs_mc"] + 0.01
+    assert r["prf_l1_9um"] < 1.5 * r["prf_l1_9um_mc_vs_mc"] + 0.01
