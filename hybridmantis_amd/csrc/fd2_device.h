@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/fd2.h"
 #include "fd2_math.h"
 
 namespace fd2 {
@@ -147,41 +148,50 @@ enum Fate : int {
   kResume = 8  // alive: reflected by the sensor, its next in-column step is due
 };
 
-// photon state, all in registers
+// photon state, all in registers.  The event's position and history stay
+// in the event record; the caller passes them where they are needed.
 struct Photon {
-  float px, py, pz;   // position
+  float px, py, pz;   // position (the detection point once detected)
   float ux, uy, uz;   // direction cosines
   float nx, ny, nz;   // surface normal
-  float xc, yc;       // current column axis
-  float ex, ey, ez;   // event position
-  float ax0;          // x of the first column's axis (event x - col_dx)
-  int32_t hist;       // 1-based history
+  float xc, yc;       // current column axis (the event's first column at start)
   bool first;         // no boundary interaction yet (num_rebound == 0)
   int32_t loops;      // loop passes so far (kMaxLoops cap)
   int fate;
   int32_t pix;
-  float fx, fy;
 };
 
+// A new photon of event ev at its emission point.  The first column is
+// centred on the event (x - col_dx, y): the reference re-centres at the
+// first dist_to_surface of a photon (main/kernel_cuda_c_ver1_0.cu:985-989),
+// and nothing reads the centre before that.
+__device__ __forceinline__ void start_photon(Photon& p, const fd2_event& ev) {
+  p.px = ev.x;
+  p.py = ev.y;
+  p.pz = ev.z;
+  p.xc = ev.x - ev.col_dx;
+  p.yc = ev.y;
+  p.first = true;
+  p.loops = 0;
+  p.fate = kNone;
+  p.pix = -1;
+  p.nx = p.ny = p.nz = 0.0f;
+}
+
 // Detection: pixel floor((x-lb)/pitch), inside iff lb <= x <= ub
-// (main/kernel_cuda_c_ver1_0.cu:3402-3419); per-history count at hist-1.
-// With null prf/hist pointers (the persistent kernel's geometry copy) only
-// the fate and pixel are recorded and the caller tallies them.
+// (main/kernel_cuda_c_ver1_0.cu:3402-3419).  Records the fate and pixel;
+// the kernels add the PRF count and the per-history count (at hist-1) when
+// the photon ends.
 __device__ __forceinline__ void tally_detect(Photon& p, const GeomDev& g) {
   p.fate = kDetectOut;
   p.pix = -1;
-  p.fx = p.px;
-  p.fy = p.py;
   int i = (int)floorf((p.px - g.lbx) / (float)g.pixel);
   int j = (int)floorf((p.py - g.lby) / (float)g.pixel);
   if (p.px <= g.ubx && p.py <= g.uby && p.px >= g.lbx && p.py >= g.lby &&
       i >= 0 && i < g.xdim && j >= 0 && j < g.ydim) {
-    int idx = i * g.ydim + j;
-    if (g.prf) atomicAdd(&g.prf[idx], 1ull);
     p.fate = kDetect;
-    p.pix = idx;
+    p.pix = i * g.ydim + j;
   }
-  if (g.hist && p.hist >= 1 && p.hist <= g.num_hist) atomicAdd(&g.hist[p.hist - 1], 1u);
 }
 
 // ------------------------------------------------------- vector helpers
@@ -302,10 +312,6 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
                                                  StepSink* sink) {
   const float H2 = g.half_h, R = g.R;
   const float t0 = p.px, t1 = p.py, t2 = p.pz;
-  if (p.first) {
-    p.xc = p.ax0;
-    p.yc = p.ey;
-  }
   const float xc = p.xc, yc = p.yc;
   const float dx0 = p.px - xc, dy0 = p.py - yc;
   const float qa = p.ux * p.ux + p.uy * p.uy;
@@ -878,11 +884,11 @@ __device__ __forceinline__ int photon_step(Rng& rng, Photon& p,
 // read before being overwritten, so only its two draws are consumed.
 template <class Rng>
 __device__ __forceinline__ void restart_trapped(Rng& rng, Photon& p,
-                                                const GeomDev& g) {
+                                                const GeomDev& g, const fd2_event& ev) {
   emit_dir(rng, p, g);
-  p.px = p.ex;
-  p.py = p.ey;
-  p.pz = p.ez;
+  p.px = ev.x;
+  p.py = ev.y;
+  p.pz = ev.z;
   if (g.beta > 0.0f) {
     (void)rng.u();
     (void)rng.u();

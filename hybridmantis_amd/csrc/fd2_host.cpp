@@ -210,13 +210,15 @@ Launch make_launch(const fd2_ctx* c, const QueueBufs& b, const fd2_event* d_ev, 
   a.events = d_ev;
   a.chunk_off = b.chunk_off;
   a.photon_off = b.photon_off;
-  a.geoms = c->d_geoms_mc;
-  a.tally = c->d_geoms;
+  a.geoms = c->d_geoms;
   a.counters = c->d_counters;
   a.cursor = c->d_cursor;
   a.records = d_records;
   a.radial = d_radial;
   a.event_base_dev = nullptr;
+  const bool queue = c->tune.wall_queue >= 0 && c->d_wave_queue != nullptr;
+  a.wave_queue = queue ? c->d_wave_queue : nullptr;
+  a.error_flag = c->d_error;
   a.n_events = n;
   a.seed = c->seed;
   a.event_base = (uint32_t)c->next_event;
@@ -231,8 +233,10 @@ Launch make_launch(const fd2_ctx* c, const QueueBufs& b, const fd2_event* d_ev, 
   L.radial = d_radial != nullptr;
   const int per_cu = c->tune.waves_per_cu / (kTransportThreads / 64);
   const size_t smem = (size_t)a.n_tiles * a.tile_side * a.tile_side * sizeof(uint32_t);
-  const int occ = transport_max_blocks_per_cu(L.multi, L.record, L.radial, smem);
+  const int occ = transport_max_blocks_per_cu(L.multi, L.record, L.radial, smem, queue);
   L.blocks = c->n_cu * (per_cu < 1 ? 1 : (per_cu > occ ? occ : per_cu));
+  if (queue && L.blocks * (kTransportThreads / 64) > c->max_waves)
+    L.blocks = c->max_waves / (kTransportThreads / 64);
   return L;
 }
 
@@ -258,6 +262,8 @@ int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
   return FD2_OK;
 }
 
+// bit 0: an event failed validation in the queue build; bit 1: a wave of the
+// persistent kernel hit its loop limit (results incomplete)
 int check_error_flag(fd2_ctx* c) {
   uint32_t flag = 0;
   FD2_HIP(hipMemcpyAsync(&flag, c->d_error, sizeof flag, hipMemcpyDeviceToHost, c->stream));
@@ -265,7 +271,7 @@ int check_error_flag(fd2_ctx* c) {
   if (flag) {
     FD2_HIP(launch_fill_u32(c->d_error, 1, c->stream));
     FD2_HIP(hipStreamSynchronize(c->stream));
-    return FD2_E_INVAL;
+    return (flag & 2u) ? FD2_E_DEVICE : FD2_E_INVAL;
   }
   return FD2_OK;
 }
@@ -305,6 +311,7 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
   c->tune.lds_tile = 0;
   c->tune.park_lanes = 32;
   c->tune.refill_lanes = 4;
+  c->tune.wall_queue = 0;
   int rc = FD2_OK;
   do {
     if (hipSetDevice(device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
@@ -337,22 +344,22 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
       rc = FD2_E_NOMEM;
       break;
     }
-    GeomDev mc[FD2_MAX_GEOMS];
-    for (int i = 0; i < n_geoms; ++i) {
-      mc[i] = c->hgeom[i];
-      mc[i].prf = nullptr;
-      mc[i].hist = nullptr;
-    }
-    if (hipMalloc(&c->d_geoms_mc, sizeof(GeomDev) * n_geoms) != hipSuccess) {
-      rc = FD2_E_NOMEM;
-      break;
-    }
     if (hipMemcpy(c->d_geoms, c->hgeom, sizeof(GeomDev) * n_geoms, hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(c->d_geoms_mc, mc, sizeof(GeomDev) * n_geoms, hipMemcpyHostToDevice) !=
-            hipSuccess) {
+        hipSuccess) {
       rc = FD2_E_DEVICE;
       break;
+    }
+    // per-wave photon queues for the largest resident grid of the queued kernel
+    {
+      int occ = 1;
+      for (int v = 0; v < 5; ++v) {
+        const int o = transport_max_blocks_per_cu(v == 1 || v == 3, v == 2 || v == 3, v == 4, 0,
+                                                  true);
+        if (o > occ) occ = o;
+      }
+      c->max_waves = c->n_cu * occ * (kTransportThreads / 64);
+      const size_t qbytes = (size_t)c->max_waves * kQWords * kQSlots * sizeof(uint32_t);
+      if (hipMalloc(&c->d_wave_queue, qbytes) != hipSuccess) { rc = FD2_E_NOMEM; break; }
     }
     if (launch_fill_u32(c->d_error, 1, c->stream) != hipSuccess) { rc = FD2_E_DEVICE; break; }
     if (launch_fill_u32(c->d_cursor, 1, c->stream) != hipSuccess) { rc = FD2_E_DEVICE; break; }
@@ -374,7 +381,7 @@ void fd2_destroy(fd2_ctx* c) {
     if (c->d_prf[i]) (void)hipFree(c->d_prf[i]);
     if (c->d_hist[i]) (void)hipFree(c->d_hist[i]);
   }
-  void* bufs[] = {c->d_geoms, c->d_geoms_mc, c->d_counters, c->d_events, c->d_chunk_off,
+  void* bufs[] = {c->d_geoms, c->d_wave_queue, c->d_counters, c->d_events, c->d_chunk_off,
                   c->d_photon_off, c->d_block_chunks, c->d_block_photons,
                   c->d_cursor, c->d_error};
   for (void* b : bufs)
@@ -413,6 +420,8 @@ int fd2_set_tuning(fd2_ctx* c, const fd2_tuning* t) {
     if (t->refill_lanes < 1 || t->refill_lanes > 64) return FD2_E_INVAL;
     n.refill_lanes = t->refill_lanes;
   }
+  if (t->wall_queue < -1 || t->wall_queue > 1) return FD2_E_INVAL;
+  n.wall_queue = t->wall_queue;
   c->tune = n;
   return FD2_OK;
 }
@@ -832,19 +841,23 @@ int fd2_autotune(fd2_ctx* c, const fd2_event* d_events, int64_t n, int32_t repea
   static const int parks[] = {8, 16, 32, 48};
   static const int tiles[] = {-1, 32, 48, 64, 96};
   static const int refills[] = {1, 4, 8, 16, 32};
+  static const int queues[] = {-1, 1};
   fd2_tuning cur = c->tune;
   float cur_ms = 0.0f;
   rc = time_one(cur, &cur_ms);
   for (int sweep = 0; sweep < 2 && !rc; ++sweep) {
-    for (int dim = 0; dim < 5 && !rc; ++dim) {
+    for (int dim = 0; dim < 6 && !rc; ++dim) {
       const int* vals = dim == 0 ? chunks : dim == 1 ? waves : dim == 2 ? parks
-                      : dim == 3 ? tiles : refills;
-      const int nv = dim == 1 ? 3 : dim == 2 ? 4 : 5;
+                      : dim == 3 ? tiles : dim == 4 ? refills : queues;
+      const int nv = dim == 1 ? 3 : dim == 2 ? 4 : dim == 5 ? 2 : 5;
       for (int i = 0; i < nv && !rc; ++i) {
         fd2_tuning t = cur;
         int32_t* f = dim == 0 ? &t.chunk : dim == 1 ? &t.waves_per_cu
-                   : dim == 2 ? &t.park_lanes : dim == 3 ? &t.lds_tile : &t.refill_lanes;
-        if (*f == vals[i] || (dim == 3 && *f == 0 && vals[i] == kTileDefault)) continue;
+                   : dim == 2 ? &t.park_lanes : dim == 3 ? &t.lds_tile
+                   : dim == 4 ? &t.refill_lanes : &t.wall_queue;
+        if (*f == vals[i] || (dim == 3 && *f == 0 && vals[i] == kTileDefault) ||
+            (dim == 5 && *f == 0 && vals[i] == 1))
+          continue;
         *f = vals[i];
         float ms = 0.0f;
         rc = time_one(t, &ms);

@@ -14,13 +14,14 @@ struct TransportArgs {
   const fd2_event* events;             // device
   const uint32_t* chunk_off;           // n_events + 1 (exclusive scan)
   const unsigned long long* photon_off;  // n_events + 1 (exclusive scan)
-  const GeomDev* geoms;                // n_geoms, null prf/hist: tallied by the kernel
-  const GeomDev* tally;                // n_geoms, with the prf/hist pointers
+  const GeomDev* geoms;                // n_geoms
   unsigned long long* counters;        // n_geoms * FD2_NCOUNT
   uint32_t* cursor;                    // work-queue cursor (zeroed by the scan)
   fd2_photon* records;                 // per-photon records (recording only)
   uint32_t* radial;                    // n_events x FD2_RADIAL_BINS (radial mode)
   const uint32_t* event_base_dev;      // if set, read the number of events[0] here
+  uint32_t* wave_queue;                // per-wave photon queues (null: park in place)
+  uint32_t* error_flag;                // bit 1: a wave's loop limit was reached
   int64_t n_events;
   uint32_t seed;
   uint32_t event_base;                 // global number of events[0]
@@ -37,6 +38,12 @@ struct TransportArgs {
 // (u32) and adds the block's counts to the PRF at the end.
 constexpr int kTileDefault = 96;
 constexpr int kTileBudget = 96 * 96;   // u32 entries per block (36 KiB)
+// per-wave photon queues of the persistent kernel (wall queue + backlog)
+constexpr int kWallCap = 128;
+constexpr int kBackCap = 64 + kWallCap;
+constexpr int kQSlots = kWallCap + kBackCap;
+constexpr int kQWords = 19;
+constexpr uint32_t kMaxIdlePasses = 256;  // passes without a step or a wall branch
 constexpr int kTileMax = 120;  // 57.6 KiB: within 64 KiB of LDS per workgroup
 
 struct QueueArgs {
@@ -76,7 +83,7 @@ hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
-int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem);
+int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem, bool queue);
 
 }  // namespace fd2
 
@@ -91,8 +98,7 @@ struct fd2_ctx {
   int n_geoms = 0;
   fd2_params params[FD2_MAX_GEOMS];
   fd2::GeomDev hgeom[FD2_MAX_GEOMS];
-  fd2::GeomDev* d_geoms = nullptr;     // with the tally pointers
-  fd2::GeomDev* d_geoms_mc = nullptr;  // null tally pointers (persistent kernel)
+  fd2::GeomDev* d_geoms = nullptr;
   unsigned long long* d_prf[FD2_MAX_GEOMS] = {};
   uint32_t* d_hist[FD2_MAX_GEOMS] = {};
   unsigned long long* d_counters = nullptr;
@@ -110,6 +116,8 @@ struct fd2_ctx {
   size_t cap_blocks = 0;
   uint32_t* d_cursor = nullptr;
   uint32_t* d_error = nullptr;
+  uint32_t* d_wave_queue = nullptr;    // kQWords x kQSlots words per resident wave
+  int max_waves = 0;
 
   uint32_t seed = 271828182u;
   int64_t next_event = 0;

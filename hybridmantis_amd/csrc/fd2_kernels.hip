@@ -171,26 +171,6 @@ __device__ __forceinline__ int fate_counter(int fate) {
   }
 }
 
-__device__ __forceinline__ void start_photon(Photon& p, const fd2_event& ev) {
-  p.ex = ev.x;
-  p.ey = ev.y;
-  p.ez = ev.z;
-  p.ax0 = ev.x - ev.col_dx;
-  p.px = ev.x;
-  p.py = ev.y;
-  p.pz = ev.z;
-  p.hist = ev.hist;
-  p.first = true;
-  p.loops = 0;
-  p.fate = kNone;
-  p.pix = -1;
-  p.fx = 0.0f;
-  p.fy = 0.0f;
-  p.nx = p.ny = p.nz = 0.0f;
-  p.xc = ev.x;
-  p.yc = ev.y;
-}
-
 // Counter-mode initial direction of photon k: photon 0 takes the driver's
 // first direction (|2u-1| > 0.01, main/hybridMANTIS_c_ver1_0.c:257-277),
 // the others the direction algo() draws at the previous photon's end
@@ -239,7 +219,128 @@ __device__ __forceinline__ int tile_origin(int dim, int side) {
   return dim > side ? (dim - side) >> 1 : 0;
 }
 
+// Tallies of an ended photon: fate counter, PRF pixel (LDS tile or HBM),
+// per-lane run of the history's detections, radial bin, record.
 template <bool MULTI, bool REC, bool RADIAL>
+__device__ __forceinline__ void finish_photon(const TransportArgs& a, const Photon& p, uint32_t gi,
+                                              uint32_t lane_ev, uint32_t lane_k, uint32_t* s_cnt,
+                                              uint32_t* s_tile, int side, int tile_area,
+                                              int32_t& run_hist, uint32_t& run_g,
+                                              uint32_t& run_cnt) {
+  atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
+  const bool det = p.fate == kDetect || p.fate == kDetectOut;
+  if (det) {
+    const GeomDev& gt = a.geoms[MULTI ? gi : 0u];
+    const int32_t hist = a.events[lane_ev].hist;
+    if (p.fate == kDetect) {
+      bool in_tile = false;
+      if ((int)gi < a.n_tiles) {
+        const int i = p.pix / gt.ydim;
+        const uint32_t ti = (uint32_t)(i - tile_origin(gt.xdim, side));
+        const uint32_t tj = (uint32_t)(p.pix - i * gt.ydim - tile_origin(gt.ydim, side));
+        if (ti < (uint32_t)side && tj < (uint32_t)side) {
+          atomicAdd(&s_tile[gi * tile_area + ti * side + tj], 1u);
+          in_tile = true;
+        }
+      }
+      if (!in_tile) atomicAdd(&gt.prf[p.pix], 1ull);
+    }
+    if (hist >= 1 && hist <= gt.num_hist) {
+      if (hist == run_hist && (!MULTI || gi == run_g)) {
+        ++run_cnt;
+      } else {
+        if (run_cnt) atomicAdd(&a.geoms[MULTI ? run_g : 0u].hist[run_hist - 1], run_cnt);
+        run_hist = hist;
+        run_g = gi;
+        run_cnt = 1;
+      }
+    }
+  }
+  if (RADIAL && det) {
+    const fd2_event& ev = a.events[lane_ev];
+    atomicAdd(&a.radial[(size_t)lane_ev * FD2_RADIAL_BINS + radial_bin(p.px - ev.x, p.py - ev.y)],
+              1u);
+  }
+  if (REC) {
+    fd2_photon rec;
+    rec.fate = p.fate;
+    rec.pix = p.pix;
+    rec.x = det ? p.px : 0.0f;
+    rec.y = det ? p.py : 0.0f;
+    a.records[a.photon_off[lane_ev] + lane_k] = rec;
+  }
+}
+
+// Per-wave photon queues in HBM (wall queue then backlog, kQSlots entries
+// of kQWords words; word j of slot s at [j * kQSlots + s], so the lanes of a
+// wave touch consecutive words).  An entry is everything a photon in flight
+// carries: position, direction, normal, column axis, loop count and flags,
+// random stream, event, geometry and photon index.
+__device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon& p,
+                                      const CounterRng& r, uint32_t ev, uint32_t gi, uint32_t k,
+                                      bool parked) {
+  uint32_t* w = wq + slot;
+  w[0 * kQSlots] = __float_as_uint(p.px);
+  w[1 * kQSlots] = __float_as_uint(p.py);
+  w[2 * kQSlots] = __float_as_uint(p.pz);
+  w[3 * kQSlots] = __float_as_uint(p.ux);
+  w[4 * kQSlots] = __float_as_uint(p.uy);
+  w[5 * kQSlots] = __float_as_uint(p.uz);
+  w[6 * kQSlots] = __float_as_uint(p.nx);
+  w[7 * kQSlots] = __float_as_uint(p.ny);
+  w[8 * kQSlots] = __float_as_uint(p.nz);
+  w[9 * kQSlots] = __float_as_uint(p.xc);
+  w[10 * kQSlots] = __float_as_uint(p.yc);
+  w[11 * kQSlots] = (uint32_t)p.loops | (p.first ? 1u << 21 : 0u) |
+                    (p.fate == kResume ? 1u << 22 : 0u) | (parked ? 1u << 23 : 0u);
+  w[12 * kQSlots] = r.s0;
+  w[13 * kQSlots] = r.s1;
+  w[14 * kQSlots] = r.s2;
+  w[15 * kQSlots] = r.s3;
+  w[16 * kQSlots] = ev;
+  w[17 * kQSlots] = gi;
+  w[18 * kQSlots] = k;
+}
+
+__device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon& p, CounterRng& r,
+                                      uint32_t& ev, uint32_t& gi, uint32_t& k, bool& parked) {
+  const uint32_t* w = wq + slot;
+  p.px = __uint_as_float(w[0 * kQSlots]);
+  p.py = __uint_as_float(w[1 * kQSlots]);
+  p.pz = __uint_as_float(w[2 * kQSlots]);
+  p.ux = __uint_as_float(w[3 * kQSlots]);
+  p.uy = __uint_as_float(w[4 * kQSlots]);
+  p.uz = __uint_as_float(w[5 * kQSlots]);
+  p.nx = __uint_as_float(w[6 * kQSlots]);
+  p.ny = __uint_as_float(w[7 * kQSlots]);
+  p.nz = __uint_as_float(w[8 * kQSlots]);
+  p.xc = __uint_as_float(w[9 * kQSlots]);
+  p.yc = __uint_as_float(w[10 * kQSlots]);
+  const uint32_t f = w[11 * kQSlots];
+  p.loops = (int32_t)(f & 0x1fffffu);
+  p.first = (f >> 21) & 1u;
+  p.fate = ((f >> 22) & 1u) ? kResume : kNone;
+  parked = (f >> 23) & 1u;
+  p.pix = -1;
+  r.s0 = w[12 * kQSlots];
+  r.s1 = w[13 * kQSlots];
+  r.s2 = w[14 * kQSlots];
+  r.s3 = w[15 * kQSlots];
+  ev = w[16 * kQSlots];
+  gi = w[17 * kQSlots];
+  k = w[18 * kQSlots];
+}
+
+// QUEUE = false: a lane whose photon reaches the wall-Fresnel branch parks
+// in place and the wave runs the branch for its parked lanes once
+// park_lanes of them wait.  QUEUE = true: the photon moves to the wave's
+// wall queue in HBM and the lane takes new work; when 64 photons wait, the
+// wave moves its stepping photons to its backlog and runs the branch with
+// all 64 lanes.  Free lanes refill from the backlog before the work queue,
+// so no more than 64 + kWallCap photons are ever in flight per wave and the
+// backlog (kBackCap) cannot overflow.  Every photon keeps its own state and
+// random stream, so the results are the same either way.
+template <bool MULTI, bool REC, bool RADIAL, bool QUEUE>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
 k_transport(TransportArgs a) {
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
@@ -269,11 +370,18 @@ k_transport(TransportArgs a) {
   Photon p;
   CounterRng rng;
   bool alive = false;
-  bool parked = false;  // waiting for the wave's next Fresnel/gap phase
+  bool parked = false;  // waiting for the wall-Fresnel branch
   uint32_t gi = 0;
-  unsigned long long rec_idx = 0;
-  int64_t lane_ev = 0;
+  uint32_t lane_ev = 0;  // the lane's event (index into a.events)
+  uint32_t lane_k = 0;   // photon index within the event
   StepSink nosink{nullptr, nullptr, 0, 0, false};
+
+  // wave queues (QUEUE): wall entries [0, n_wall), backlog [0, n_back)
+  uint32_t* wq = nullptr;
+  uint32_t n_wall = 0, n_back = 0, idle = 0;
+  if (QUEUE)
+    wq = a.wave_queue + (size_t)(blockIdx.x * (kTransportThreads / 64) + (threadIdx.x >> 6)) *
+                            (kQWords * kQSlots);
 
   for (;;) {
     // Starting a photon (stream seeding, first direction) is long and would
@@ -281,6 +389,16 @@ k_transport(TransportArgs a) {
     // refill_lanes lanes are free, or no lane is left to step.
     unsigned long long need = __ballot(!alive);
     if (__popcll(need) < a.refill_lanes && need != ~0ull) need = 0ull;
+    if (QUEUE && need != 0ull && n_back != 0u) {
+      const uint32_t take = min((uint32_t)__popcll(need), n_back);
+      const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+      if (!alive && rank < take) {
+        q_get(wq, kWallCap + n_back - 1u - rank, p, rng, lane_ev, gi, lane_k, parked);
+        alive = true;
+      }
+      n_back -= take;
+      need = __ballot(!alive);
+    }
     while (need != 0ull && !pool_done) {
       if (pool_next >= pool_end) {
         uint32_t c = 0;
@@ -315,15 +433,21 @@ k_transport(TransportArgs a) {
         start_photon(p, ev);
         rng.seed(a.seed, event_base + (uint32_t)pool_ev, k);
         counter_initial_dir(rng, p, k, g);
-        if (REC) rec_idx = a.photon_off[pool_ev] + k;
-        if (RADIAL) lane_ev = pool_ev;
+        lane_ev = (uint32_t)pool_ev;
+        lane_k = k;
+        parked = false;
         alive = true;
       }
       pool_next += min(__popcll(need), avail);
       need = __ballot(!alive);
     }
-    if (__ballot(alive) == 0ull) break;
+    // with no lane alive the work queue and the backlog are empty
+    if (__ballot(alive) == 0ull && (!QUEUE || n_wall == 0u)) break;
     bool ended = false;
+    // every pass steps a photon or runs the wall branch; a wave that does
+    // neither for many passes has a broken queue state: it stops and flags
+    // the launch rather than spin
+    bool progress = __ballot(alive && !parked) != 0ull;
     if (alive && !parked) {
       const GeomDev& g = a.geoms[MULTI ? gi : 0u];
       const int r = hot_step<CounterRng, false>(rng, p, g, &nosink);
@@ -331,70 +455,99 @@ k_transport(TransportArgs a) {
         ended = true;
       } else if (r == kTrapped) {
         atomicAdd(&s_cnt[gi * kCounters + FD2_C_REGEN], 1u);
-        restart_trapped(rng, p, g);
+        restart_trapped(rng, p, g, a.events[lane_ev]);
       } else if (r == kNeedPrpt) {
         parked = true;
       }
     }
-    // The Fresnel/gap branch is long and taken by few lanes at a time: run
-    // it once enough lanes wait for it, or when nothing else can run.
-    const unsigned long long parked_mask = __ballot(parked);
-    if (parked_mask != 0ull &&
-        (__popcll(parked_mask) >= a.park_lanes || __ballot(alive && !parked) == 0ull)) {
-      if (parked) {
-        const GeomDev& g = a.geoms[MULTI ? gi : 0u];
-        ended = ba_prpt(rng, p, g);
-        parked = false;
+    if (QUEUE) {
+      if (ended) {
+        finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+                                          tile_area, run_hist, run_g, run_cnt);
+        alive = false;
+        ended = false;
       }
-    }
-    if (ended) {
-      atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
-      if (p.fate == kDetect || p.fate == kDetectOut) {
-        const GeomDev& gt = a.tally[MULTI ? gi : 0u];
-        if (p.fate == kDetect) {
-          bool in_tile = false;
-          if ((int)gi < a.n_tiles) {
-            const int i = p.pix / gt.ydim;
-            const uint32_t ti = (uint32_t)(i - tile_origin(gt.xdim, side));
-            const uint32_t tj = (uint32_t)(p.pix - i * gt.ydim - tile_origin(gt.ydim, side));
-            if (ti < (uint32_t)side && tj < (uint32_t)side) {
-              atomicAdd(&s_tile[gi * tile_area + ti * side + tj], 1u);
-              in_tile = true;
-            }
-          }
-          if (!in_tile) atomicAdd(&gt.prf[p.pix], 1ull);
+      // parked photons move to the wall queue while it has room
+      const unsigned long long pm = __ballot(alive && parked);
+      if (pm != 0ull) {
+        const uint32_t k = min((uint32_t)__popcll(pm), (uint32_t)kWallCap - n_wall);
+        const uint32_t rank = (uint32_t)__popcll(pm & lt_mask);
+        if (alive && parked && rank < k) {
+          q_put(wq, n_wall + rank, p, rng, lane_ev, gi, lane_k, false);
+          alive = false;
+          parked = false;
         }
-        if (p.hist >= 1 && p.hist <= gt.num_hist) {
-          if (p.hist == run_hist && (!MULTI || gi == run_g)) {
-            ++run_cnt;
-          } else {
-            if (run_cnt) atomicAdd(&a.tally[MULTI ? run_g : 0u].hist[run_hist - 1], run_cnt);
-            run_hist = p.hist;
-            run_g = gi;
-            run_cnt = 1;
-          }
+        n_wall += k;
+        __threadfence_block();
+      }
+      const bool drain = n_wall != 0u && __ballot(alive) == 0ull && pool_done && n_back == 0u;
+      if (n_wall >= 64u || drain) {
+        progress = true;
+        // stepping photons to the backlog, then up to 64 from the wall queue
+        const unsigned long long am = __ballot(alive);
+        const uint32_t m = (uint32_t)__popcll(am);
+        if (n_back + m > (uint32_t)kBackCap) {  // cannot happen (see above)
+          if (lane == 0) atomicOr(a.error_flag, 2u);
+          break;
+        }
+        if (alive) {
+          q_put(wq, kWallCap + n_back + (uint32_t)__popcll(am & lt_mask), p, rng, lane_ev, gi,
+                lane_k, parked);
+          alive = false;
+          parked = false;
+        }
+        n_back += m;
+        __threadfence_block();
+        const uint32_t take = min(n_wall, 64u);
+        if ((uint32_t)lane < take) {
+          bool pk;
+          q_get(wq, n_wall - take + (uint32_t)lane, p, rng, lane_ev, gi, lane_k, pk);
+          alive = true;
+        }
+        n_wall -= take;
+        if (alive) {
+          const GeomDev& g = a.geoms[MULTI ? gi : 0u];
+          ended = ba_prpt(rng, p, g);
+        }
+        if (ended) {
+          finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+                                            tile_area, run_hist, run_g, run_cnt);
+          alive = false;
         }
       }
-      if (RADIAL && (p.fate == kDetect || p.fate == kDetectOut))
-        atomicAdd(&a.radial[lane_ev * FD2_RADIAL_BINS + radial_bin(p.fx - p.ex, p.fy - p.ey)], 1u);
-      if (REC) {
-        fd2_photon rec;
-        rec.fate = p.fate;
-        rec.pix = p.pix;
-        rec.x = p.fx;
-        rec.y = p.fy;
-        a.records[rec_idx] = rec;
+      if (progress) {
+        idle = 0;
+      } else if (++idle > kMaxIdlePasses) {
+        if (lane == 0) atomicOr(a.error_flag, 2u);
+        break;
       }
-      alive = false;
+    } else {
+      // The wall-Fresnel branch (with the gap flights it leads to) is long
+      // and taken by few lanes at a time: run it once enough lanes wait for
+      // it, or when nothing else can run.
+      const unsigned long long parked_mask = __ballot(parked);
+      if (parked_mask != 0ull &&
+          (__popcll(parked_mask) >= a.park_lanes || __ballot(alive && !parked) == 0ull)) {
+        if (parked) {
+          const GeomDev& g = a.geoms[MULTI ? gi : 0u];
+          ended = ba_prpt(rng, p, g);
+          parked = false;
+        }
+      }
+      if (ended) {
+        finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+                                          tile_area, run_hist, run_g, run_cnt);
+        alive = false;
+      }
     }
   }
 
-  if (run_cnt) atomicAdd(&a.tally[MULTI ? run_g : 0u].hist[run_hist - 1], run_cnt);
+  if (run_cnt) atomicAdd(&a.geoms[MULTI ? run_g : 0u].hist[run_hist - 1], run_cnt);
   __syncthreads();
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x)
     if (s_cnt[i]) atomicAdd(&a.counters[i], (unsigned long long)s_cnt[i]);
   for (int t = 0; t < a.n_tiles; ++t) {
-    const GeomDev& gt = a.tally[t];
+    const GeomDev& gt = a.geoms[t];
     const int i0 = tile_origin(gt.xdim, side), j0 = tile_origin(gt.ydim, side);
     for (int e = threadIdx.x; e < tile_area; e += blockDim.x) {
       const uint32_t v = s_tile[t * tile_area + e];
@@ -410,8 +563,12 @@ k_transport(TransportArgs a) {
 template <bool MULTI, bool REC, bool RADIAL>
 static void launch_one(const TransportArgs& a, dim3 grid, hipStream_t s) {
   const size_t smem = (size_t)a.n_tiles * a.tile_side * a.tile_side * sizeof(uint32_t);
-  hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL>), grid, dim3(kTransportThreads), smem, s,
-                     a);
+  if (a.wave_queue)
+    hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL, true>), grid, dim3(kTransportThreads),
+                       smem, s, a);
+  else
+    hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL, false>), grid, dim3(kTransportThreads),
+                       smem, s, a);
 }
 
 hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
@@ -431,20 +588,26 @@ hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
 }
 
 template <bool MULTI, bool REC, bool RADIAL>
-static int occupancy_of(size_t smem) {
+static int occupancy_of(size_t smem, bool queue) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_transport<MULTI, REC, RADIAL>,
-                                                   kTransportThreads, smem) != hipSuccess)
-    return 1;
+  const hipError_t e =
+      queue ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &nb, k_transport<MULTI, REC, RADIAL, true>, kTransportThreads, smem)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &nb, k_transport<MULTI, REC, RADIAL, false>, kTransportThreads, smem);
+  if (e != hipSuccess) return 1;
   return nb > 0 ? nb : 1;
 }
 
-int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem) {
+int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem, bool queue) {
   if (radial)
-    return multi ? occupancy_of<true, false, true>(smem) : occupancy_of<false, false, true>(smem);
+    return multi ? occupancy_of<true, false, true>(smem, queue)
+                 : occupancy_of<false, false, true>(smem, queue);
   if (multi)
-    return record ? occupancy_of<true, true, false>(smem) : occupancy_of<true, false, false>(smem);
-  return record ? occupancy_of<false, true, false>(smem) : occupancy_of<false, false, false>(smem);
+    return record ? occupancy_of<true, true, false>(smem, queue)
+                  : occupancy_of<true, false, false>(smem, queue);
+  return record ? occupancy_of<false, true, false>(smem, queue)
+                : occupancy_of<false, false, false>(smem, queue);
 }
 
 // ------------------------------------------------ RANECU trajectory mode
@@ -465,29 +628,30 @@ __global__ void __launch_bounds__(64) k_trace_ranecu(
   start_photon(p, ev);
   if (ev.N > 0) emit_first_dir(rng, p, g);  // driver, normal stays 0
   for (int32_t k = 0; k < ev.N; ++k) {
-    p.px = ev.x;
-    p.py = ev.y;
-    p.pz = ev.z;
-    p.first = true;
-    p.loops = 0;
-    p.fate = kNone;
-    p.pix = -1;
+    // direction and normal carry over from the previous photon's end
+    const float ux = p.ux, uy = p.uy, uz = p.uz, nx = p.nx, ny = p.ny, nz = p.nz;
+    start_photon(p, ev);
+    p.ux = ux; p.uy = uy; p.uz = uz;
+    p.nx = nx; p.ny = ny; p.nz = nz;
     cnt[FD2_C_GENERATED]++;
     for (;;) {
       const int r = photon_step<RanecuRng, false>(rng, p, g, &nosink);
       if (r == kEnded) break;
       if (r == kTrapped) {
         cnt[FD2_C_REGEN]++;
-        restart_trapped(rng, p, g);
+        restart_trapped(rng, p, g, ev);
       }
     }
     cnt[fate_counter(p.fate)]++;
+    const bool det = p.fate == kDetect || p.fate == kDetectOut;
+    if (p.fate == kDetect) atomicAdd(&g.prf[p.pix], 1ull);
+    if (det && ev.hist >= 1 && ev.hist <= g.num_hist) atomicAdd(&g.hist[ev.hist - 1], 1u);
     if (records) {
       fd2_photon rec;
       rec.fate = p.fate;
       rec.pix = p.pix;
-      rec.x = p.fx;
-      rec.y = p.fy;
+      rec.x = det ? p.px : 0.0f;
+      rec.y = det ? p.py : 0.0f;
       records[photon_off[i] + k] = rec;
     }
     // next photon's direction and normal draws (main/kernel_cuda_c_ver1_0.cu:578-602)
@@ -535,7 +699,7 @@ __global__ void __launch_bounds__(64) k_debug_steps(
   for (;;) {
     const int r = photon_step<CounterRng, true>(rng, p, g, &sink);
     if (r == kEnded) break;
-    if (r == kTrapped) restart_trapped(rng, p, g);
+    if (r == kTrapped) restart_trapped(rng, p, g, ev);
   }
 }
 

@@ -49,7 +49,7 @@ class Params(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("waves_per_cu", C.c_int32),
                 ("lds_tile", C.c_int32), ("park_lanes", C.c_int32),
-                ("refill_lanes", C.c_int32)]
+                ("refill_lanes", C.c_int32), ("wall_queue", C.c_int32)]
 
 
 # dict keys (events.BASE_PARAMS) -> fd2_params members
@@ -235,15 +235,16 @@ class Transport:
 
     # -- configuration
     def set_tuning(self, chunk: int = 0, waves_per_cu: int = 0, lds_tile: int = 0,
-                   park_lanes: int = 0, refill_lanes: int = 0):
-        t = Tuning(chunk, waves_per_cu, lds_tile, park_lanes, refill_lanes)
+                   park_lanes: int = 0, refill_lanes: int = 0, wall_queue: int = 0):
+        t = Tuning(chunk, waves_per_cu, lds_tile, park_lanes, refill_lanes, wall_queue)
         _check(lib().fd2_set_tuning(self._ctx, C.byref(t)), "set_tuning")
 
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().fd2_get_tuning(self._ctx, C.byref(t)), "get_tuning")
         return dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu, lds_tile=t.lds_tile,
-                    park_lanes=t.park_lanes, refill_lanes=t.refill_lanes)
+                    park_lanes=t.park_lanes, refill_lanes=t.refill_lanes,
+                    wall_queue=t.wall_queue)
 
     def set_seed(self, seed: int) -> None:
         _check(lib().fd2_set_seed(self._ctx, seed & 0xFFFFFFFF), "set_seed")
@@ -312,7 +313,7 @@ class Transport:
                                   max_results, C.byref(nres)), "fd2_autotune")
         as_dict = lambda t: dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu,  # noqa: E731
                                  lds_tile=t.lds_tile, park_lanes=t.park_lanes,
-                                 refill_lanes=t.refill_lanes)
+                                 refill_lanes=t.refill_lanes, wall_queue=t.wall_queue)
         return as_dict(best), [(as_dict(r.tuning), float(r.ms)) for r in res[:nres.value]]
 
     # -- tallies

@@ -115,6 +115,10 @@ typedef struct fd2_tuning {
                              * inter-column phase before it runs (1-64)     */
   int32_t refill_lanes;     /* free lanes of a wave before new photons are
                              * started (1-64)                               */
+  int32_t wall_queue;       /* photons waiting for the wall-Fresnel branch:
+                             * 1 = move to a per-wave queue (the branch runs
+                             * with 64 lanes), -1 = park in their lanes
+                             * (park_lanes applies); 0 = default (1)        */
 } fd2_tuning;
 
 typedef struct fd2_ctx fd2_ctx;
@@ -293,8 +297,8 @@ void fd2_graph_destroy(fd2_graph *graph);
  * scratch context with the same parameters (the context's tallies are not
  * touched) for candidate shapes -- coordinate descent, two sweeps over chunk
  * {16..256}, waves per CU {8, 12, 16}, parked lanes {8..48}, LDS tile side
- * {none, 32..96} and refill lanes {1..32}, each the best of `repeats` runs
- * after a warm-up --
+ * {none, 32..96}, refill lanes {1..32} and wall queue {off, on}, each the
+ * best of `repeats` runs after a warm-up --
  * and sets the fastest on ctx.  Candidates and their times go to results
  * (up to max_results; *n_results = number written). */
 typedef struct fd2_tune_result {

@@ -78,7 +78,8 @@ def test_mixed_geometries_equal_separate_runs(fd2lib, batch):
 
 def test_tallies_do_not_depend_on_launch_shape(fd2lib, batch):
     plist, ev = batch
-    shapes = [dict(), dict(chunk=16, waves_per_cu=8, lds_tile=-1, park_lanes=8, refill_lanes=1),
+    shapes = [dict(), dict(wall_queue=-1),
+              dict(chunk=16, waves_per_cu=8, lds_tile=-1, park_lanes=8, refill_lanes=1),
               dict(chunk=256, waves_per_cu=12, lds_tile=32, park_lanes=64, refill_lanes=64),
               dict(chunk=32, waves_per_cu=16, lds_tile=120, park_lanes=1, refill_lanes=17)]
     ref = None
@@ -104,7 +105,8 @@ def test_autotuner(fd2lib, batch):
         start_ms = results[0][1]
         best_ms = min(ms for _, ms in results)
         assert any(t == best for t, _ in results)
-        assert [ms for t, ms in results if t == best][0] == best_ms <= start_ms
+        # a shape can be timed twice (coordinate descent may come back to it)
+        assert min(ms for t, ms in results if t == best) == best_ms <= start_ms
         # the autotuner runs on a scratch context: no tallies here yet
         assert tr.counters(0)["generated"] == 0
         with tr.graph(buf, len(ev)) as g:

@@ -31,3 +31,4 @@ def test_domains(oracle):
     assert oracle.math_eval("acos", [1.0])[0] == 0.0
     y = oracle.math_eval("acos", [-1.0])[0]
     assert abs(y - np.pi) < 3e-7
+
