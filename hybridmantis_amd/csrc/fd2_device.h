@@ -719,20 +719,39 @@ __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
 }
 
 // Fresnel reflection/refraction at a column wall and the flights between
-// columns it leads to (K:1677-1955).  true = ended.
+// columns it leads to (K:1677-1955).
+//
+// The reference runs this as one goto loop whose locals persist across its
+// passes: the resample/retry counters, the stale Pr/Pt of the gap side
+// without Fresnel recomputation, and the stale theta2 that a refraction
+// then uses.  PrptState holds them so the loop can stop after the
+// refraction out of a column (kPrGap) and continue later with the flight
+// through the gap (inside = false); the photon's arithmetic and draws are
+// the same whether or not it stops there.
+struct PrptState {
+  uint32_t tries;  // theta, smooth, reperturb, normal counters, 8 bits each
+  float pr, pt, t2;
+};
+enum PrptResult : int { kPrAlive = 0, kPrEnded = 1, kPrGap = 2 };
+
 template <class Rng>
-__device__ __forceinline__ bool ba_prpt(Rng& rng, Photon& p, const GeomDev& g) {
+__device__ __forceinline__ int prpt_loop(Rng& rng, Photon& p, const GeomDev& g, PrptState& st,
+                                         bool inside, bool stop_at_gap) {
   const float R = g.R;
-  bool inside = true;  // flag_call_transmit
-  int theta_tries = 0, smooth_tries = 0, reperturb = 0, normal_tries = 0;
-  float pr = 0.0f, pt = 0.0f, t1 = 0.0f, t2 = 0.0f;
+  int theta_tries = (int)(st.tries & 0xffu), smooth_tries = (int)((st.tries >> 8) & 0xffu);
+  int reperturb = (int)((st.tries >> 16) & 0xffu), normal_tries = (int)(st.tries >> 24);
+  float pr = st.pr, pt = st.pt, t1 = 0.0f, t2 = st.t2;
   float onx, ony, onz, odx, ody, odz;
   float mx, my, mz;
+
+  if (!inside) {
+    if (transmit_gap(rng, p, g)) return kPrEnded;
+  }
 
 prpt:
   if (++p.loops > kMaxLoops) {
     p.fate = kLost;
-    return true;
+    return kPrEnded;
   }
   if (inside) {
     const float ax = p.xc - p.px, ay = p.yc - p.py;
@@ -764,14 +783,14 @@ no_perturbation:
   t1 = mx * p.nx + my * p.ny + mz * p.nz;
   if (t1 > 1.0f || t1 < 0.0f) {
     if (!inside) {
-      if (column_normal(rng, p, g, mx, my, mz, normal_tries, false)) return true;
+      if (column_normal(rng, p, g, mx, my, mz, normal_tries, false)) return kPrEnded;
     }
     if (theta_tries < 100) {
       theta_tries++;
       goto prpt;
     }
     p.fate = kTheta1;
-    return true;
+    return kPrEnded;
   }
   t1 = fd2_acos(t1);
 
@@ -818,21 +837,36 @@ no_perturbation:
         goto no_perturbation;
       }
       p.fate = kTheta1;
-      return true;
+      return kPrEnded;
     }
     if (!inside) {
-      if (transmit_gap(rng, p, g)) return true;
+      if (transmit_gap(rng, p, g)) return kPrEnded;
       goto prpt;
     }
-    return false;
+    return kPrAlive;
   }
   refract(p, t1, t2, g);
   if (inside) {
     inside = false;
-    if (transmit_gap(rng, p, g)) return true;
+    if (stop_at_gap) {
+      st.tries = (uint32_t)theta_tries | ((uint32_t)smooth_tries << 8) |
+                 ((uint32_t)reperturb << 16) | ((uint32_t)normal_tries << 24);
+      st.pr = pr;
+      st.pt = pt;
+      st.t2 = t2;
+      return kPrGap;
+    }
+    if (transmit_gap(rng, p, g)) return kPrEnded;
     goto prpt;
   }
-  return false;
+  return kPrAlive;
+}
+
+// The whole branch in one call.  true = ended.
+template <class Rng>
+__device__ __forceinline__ bool ba_prpt(Rng& rng, Photon& p, const GeomDev& g) {
+  PrptState st{0u, 0.0f, 0.0f, 0.0f};
+  return prpt_loop(rng, p, g, st, true, false) == kPrEnded;
 }
 
 // One algo() pass of the reference (main/kernel_cuda_c_ver1_0.cu:553-655)

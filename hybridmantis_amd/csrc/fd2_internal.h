@@ -38,11 +38,15 @@ struct TransportArgs {
 // (u32) and adds the block's counts to the PRF at the end.
 constexpr int kTileDefault = 96;
 constexpr int kTileBudget = 96 * 96;   // u32 entries per block (36 KiB)
-// per-wave photon queues of the persistent kernel (wall queue + backlog)
+// per-wave photon queues of the persistent kernel: wall queue, gap queue,
+// backlog (slots in that order)
 constexpr int kWallCap = 128;
-constexpr int kBackCap = 64 + kWallCap;
-constexpr int kQSlots = kWallCap + kBackCap;
-constexpr int kQWords = 19;
+constexpr int kGapCap = 128;
+constexpr int kBackCap = 64 + kWallCap + kGapCap;
+constexpr int kGapBase = kWallCap;
+constexpr int kBackBase = kWallCap + kGapCap;
+constexpr int kQSlots = kWallCap + kGapCap + kBackCap;
+constexpr int kQWords = 23;
 constexpr uint32_t kMaxIdlePasses = 256;  // passes without a step or a wall branch
 constexpr int kTileMax = 120;  // 57.6 KiB: within 64 KiB of LDS per workgroup
 

@@ -271,11 +271,12 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Phot
   }
 }
 
-// Per-wave photon queues in HBM (wall queue then backlog, kQSlots entries
-// of kQWords words; word j of slot s at [j * kQSlots + s], so the lanes of a
-// wave touch consecutive words).  An entry is everything a photon in flight
-// carries: position, direction, normal, column axis, loop count and flags,
-// random stream, event, geometry and photon index.
+// Per-wave photon queues in HBM (wall queue, gap queue, backlog: kQSlots
+// entries of kQWords words; word j of slot s at [j * kQSlots + s], so the
+// lanes of a wave touch consecutive words).  An entry is everything a photon
+// in flight carries: position, direction, normal, column axis, loop count
+// and flags, random stream, event, geometry and photon index; gap-queue
+// entries also the wall branch's loop state (PrptState).
 __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon& p,
                                       const CounterRng& r, uint32_t ev, uint32_t gi, uint32_t k,
                                       bool parked) {
@@ -300,6 +301,22 @@ __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon&
   w[16 * kQSlots] = ev;
   w[17 * kQSlots] = gi;
   w[18 * kQSlots] = k;
+}
+
+__device__ __forceinline__ void q_put_state(uint32_t* wq, uint32_t slot, const PrptState& st) {
+  uint32_t* w = wq + slot;
+  w[19 * kQSlots] = st.tries;
+  w[20 * kQSlots] = __float_as_uint(st.pr);
+  w[21 * kQSlots] = __float_as_uint(st.pt);
+  w[22 * kQSlots] = __float_as_uint(st.t2);
+}
+
+__device__ __forceinline__ void q_get_state(const uint32_t* wq, uint32_t slot, PrptState& st) {
+  const uint32_t* w = wq + slot;
+  st.tries = w[19 * kQSlots];
+  st.pr = __uint_as_float(w[20 * kQSlots]);
+  st.pt = __uint_as_float(w[21 * kQSlots]);
+  st.t2 = __uint_as_float(w[22 * kQSlots]);
 }
 
 __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon& p, CounterRng& r,
@@ -331,15 +348,40 @@ __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon&
   k = w[18 * kQSlots];
 }
 
+// Moves the wave's photons in flight (stepping or waiting for the wall
+// queue) to the top of its backlog; false if the backlog would overflow,
+// which the in-flight bound rules out.
+__device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, bool& alive,
+                                                bool& parked, const Photon& p,
+                                                const CounterRng& rng, uint32_t lane_ev,
+                                                uint32_t gi, uint32_t lane_k,
+                                                unsigned long long lt_mask) {
+  const unsigned long long am = __ballot(alive);
+  const uint32_t m = (uint32_t)__popcll(am);
+  if (n_back + m > (uint32_t)kBackCap) return false;
+  if (alive) {
+    q_put(wq, kBackBase + n_back + (uint32_t)__popcll(am & lt_mask), p, rng, lane_ev, gi, lane_k,
+          parked);
+    alive = false;
+    parked = false;
+  }
+  n_back += m;
+  __threadfence_block();
+  return true;
+}
+
 // QUEUE = false: a lane whose photon reaches the wall-Fresnel branch parks
 // in place and the wave runs the branch for its parked lanes once
 // park_lanes of them wait.  QUEUE = true: the photon moves to the wave's
 // wall queue in HBM and the lane takes new work; when 64 photons wait, the
-// wave moves its stepping photons to its backlog and runs the branch with
-// all 64 lanes.  Free lanes refill from the backlog before the work queue,
-// so no more than 64 + kWallCap photons are ever in flight per wave and the
-// backlog (kBackCap) cannot overflow.  Every photon keeps its own state and
-// random stream, so the results are the same either way.
+// wave moves its photons in flight to its backlog and runs the branch with
+// all 64 lanes, up to a refraction out of the column; those photons wait in
+// the gap queue and their flights through the gap run, again 64 at a time,
+// in a phase of their own.  Free lanes refill from the backlog before the
+// work queue, so no more than 64 + kWallCap + kGapCap photons are ever in
+// flight per wave and the backlog (kBackCap) cannot overflow; the gap queue
+// holds at most 63 + 64.  Every photon keeps its own state and random
+// stream, so the results are the same either way.
 template <bool MULTI, bool REC, bool RADIAL, bool QUEUE>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
 k_transport(TransportArgs a) {
@@ -376,9 +418,9 @@ k_transport(TransportArgs a) {
   uint32_t lane_k = 0;   // photon index within the event
   StepSink nosink{nullptr, nullptr, 0, 0, false};
 
-  // wave queues (QUEUE): wall entries [0, n_wall), backlog [0, n_back)
+  // wave queues (QUEUE): wall [0, n_wall), gap [0, n_gap), backlog [0, n_back)
   uint32_t* wq = nullptr;
-  uint32_t n_wall = 0, n_back = 0, idle = 0;
+  uint32_t n_wall = 0, n_gap = 0, n_back = 0, idle = 0;
   if (QUEUE)
     wq = a.wave_queue + (size_t)(blockIdx.x * (kTransportThreads / 64) + (threadIdx.x >> 6)) *
                             (kQWords * kQSlots);
@@ -393,7 +435,7 @@ k_transport(TransportArgs a) {
       const uint32_t take = min((uint32_t)__popcll(need), n_back);
       const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
       if (!alive && rank < take) {
-        q_get(wq, kWallCap + n_back - 1u - rank, p, rng, lane_ev, gi, lane_k, parked);
+        q_get(wq, kBackBase + n_back - 1u - rank, p, rng, lane_ev, gi, lane_k, parked);
         alive = true;
       }
       n_back -= take;
@@ -442,7 +484,7 @@ k_transport(TransportArgs a) {
       need = __ballot(!alive);
     }
     // with no lane alive the work queue and the backlog are empty
-    if (__ballot(alive) == 0ull && (!QUEUE || n_wall == 0u)) break;
+    if (__ballot(alive) == 0ull && (!QUEUE || (n_wall == 0u && n_gap == 0u))) break;
     bool ended = false;
     // every pass steps a photon or runs the wall branch; a wave that does
     // neither for many passes has a broken queue state: it stops and flags
@@ -480,24 +522,14 @@ k_transport(TransportArgs a) {
         n_wall += k;
         __threadfence_block();
       }
-      const bool drain = n_wall != 0u && __ballot(alive) == 0ull && pool_done && n_back == 0u;
-      if (n_wall >= 64u || drain) {
+      // With every lane idle and no new work, the queues drain: wall first.
+      const bool lanes_idle = __ballot(alive) == 0ull && pool_done && n_back == 0u;
+      if (n_wall >= 64u || (lanes_idle && n_wall != 0u)) {
         progress = true;
-        // stepping photons to the backlog, then up to 64 from the wall queue
-        const unsigned long long am = __ballot(alive);
-        const uint32_t m = (uint32_t)__popcll(am);
-        if (n_back + m > (uint32_t)kBackCap) {  // cannot happen (see above)
+        if (!swap_to_backlog(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k, lt_mask)) {
           if (lane == 0) atomicOr(a.error_flag, 2u);
           break;
         }
-        if (alive) {
-          q_put(wq, kWallCap + n_back + (uint32_t)__popcll(am & lt_mask), p, rng, lane_ev, gi,
-                lane_k, parked);
-          alive = false;
-          parked = false;
-        }
-        n_back += m;
-        __threadfence_block();
         const uint32_t take = min(n_wall, 64u);
         if ((uint32_t)lane < take) {
           bool pk;
@@ -505,14 +537,53 @@ k_transport(TransportArgs a) {
           alive = true;
         }
         n_wall -= take;
-        if (alive) {
-          const GeomDev& g = a.geoms[MULTI ? gi : 0u];
-          ended = ba_prpt(rng, p, g);
-        }
-        if (ended) {
+        PrptState st{0u, 0.0f, 0.0f, 0.0f};
+        int r = kPrAlive;
+        if (alive) r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, true, true);
+        if (alive && r == kPrEnded) {
           finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                             tile_area, run_hist, run_g, run_cnt);
           alive = false;
+        }
+        // refracted out of the column: the gap flight waits in the gap queue
+        // (it holds at most 63 + 64 entries here: the gap phase below runs
+        // whenever 64 wait)
+        const unsigned long long gm = __ballot(alive && r == kPrGap);
+        if (gm != 0ull) {
+          if (alive && r == kPrGap) {
+            const uint32_t slot = kGapBase + n_gap + (uint32_t)__popcll(gm & lt_mask);
+            q_put(wq, slot, p, rng, lane_ev, gi, lane_k, false);
+            q_put_state(wq, slot, st);
+            alive = false;
+          }
+          n_gap += (uint32_t)__popcll(gm);
+          __threadfence_block();
+        }
+      }
+      if (n_gap >= 64u || (n_gap != 0u && n_wall == 0u && __ballot(alive) == 0ull && pool_done &&
+                           n_back == 0u)) {
+        progress = true;
+        if (!swap_to_backlog(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k, lt_mask)) {
+          if (lane == 0) atomicOr(a.error_flag, 2u);
+          break;
+        }
+        const uint32_t take = min(n_gap, 64u);
+        PrptState st{0u, 0.0f, 0.0f, 0.0f};
+        if ((uint32_t)lane < take) {
+          bool pk;
+          const uint32_t slot = kGapBase + n_gap - take + (uint32_t)lane;
+          q_get(wq, slot, p, rng, lane_ev, gi, lane_k, pk);
+          q_get_state(wq, slot, st);
+          alive = true;
+        }
+        n_gap -= take;
+        if (alive) {
+          const int r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, false, false);
+          if (r == kPrEnded) {
+            finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+                                              tile_area, run_hist, run_g, run_cnt);
+            alive = false;
+          }
         }
       }
       if (progress) {
