@@ -20,9 +20,8 @@
 
 namespace fd2 {
 
-// float forms of the reference's constants (main/kernel_cuda_c_ver1_0.cu:56-58)
-constexpr float kTwoPi = 6.283185308f;
-constexpr float kPi = 3.14159265f;
+// The reference's azimuths are u * twopipen (main/kernel_cuda_c_ver1_0.cu:56);
+// here they are the turn u itself (fd2_sincos_turns).
 // The reference compares floats against the double constant eps =
 // 8.1929093e-6 (and 1 +- eps, R - 1e-5, asin(nIC/nC)).  Each such test is
 // done here against the float threshold that gives the same answer for every
@@ -238,16 +237,15 @@ __device__ __forceinline__ void rough(Rng& rng, float& nx, float& ny, float& nz,
   float qz = 2.0f * s - 1.0f;
   float rr = sqrtf(1.0f - s * s);
   s = rng.u();
-  float th = s * 2.0f * kPi;
   float sn, cs;
-  fd2_sincos(th, &sn, &cs);
+  fd2_sincos_turns(s, &sn, &cs);  // azimuth 2 pi s
   float qx = rr * cs, qy = rr * sn;
-  float inv = 1.0f / sqrtf(qx * qx + qy * qy + qz * qz);
+  float inv = fd2_rsqrt(qx * qx + qy * qy + qz * qz);
   qx = qx * inv;
   qy = qy * inv;
   qz = qz * inv;
   float rx = beta * qx + nx, ry = beta * qy + ny, rz = beta * qz + nz;
-  inv = 1.0f / sqrtf(rx * rx + ry * ry + rz * rz);
+  inv = fd2_rsqrt(rx * rx + ry * ry + rz * rz);
   nx = rx * inv;
   ny = ry * inv;
   nz = rz * inv;
@@ -260,9 +258,8 @@ __device__ __forceinline__ void emit_dir(Rng& rng, Photon& p, const GeomDev& g) 
   float r = rng.u() * 2.0f - 1.0f;
   p.uz = r;
   float rr = sqrtf(1.0f - r * r);
-  float th = rng.u() * kTwoPi;
   float sn, cs;
-  fd2_sincos(th, &sn, &cs);
+  fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
   p.ux = rr * cs;
   p.uy = rr * sn;
   renorm(p.ux, p.uy, p.uz, g);
@@ -277,9 +274,8 @@ __device__ __forceinline__ void emit_first_dir(Rng& rng, Photon& p,
   while (fabsf(r) <= 0.01f) r = rng.u() * 2.0f - 1.0f;
   p.uz = r;
   float rr = sqrtf(1.0f - r * r);
-  float th = rng.u() * kTwoPi;
   float sn, cs;
-  fd2_sincos(th, &sn, &cs);
+  fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
   p.ux = rr * cs;
   p.uy = rr * sn;
   renorm(p.ux, p.uy, p.uz, g);
@@ -481,7 +477,7 @@ __device__ __forceinline__ bool column_normal(Rng& rng, Photon& p,
   for (;;) {
     float nx = mx, ny = my, nz = mz;
     rough(rng, nx, ny, nz, 1.0f);
-    const float inv = 1.0f / sqrtf(nx * nx + ny * ny);
+    const float inv = fd2_rsqrt(nx * nx + ny * ny);
     nz = 0.0f;
     nx = nx * inv;
     ny = ny * inv;
@@ -613,9 +609,8 @@ top:
     p.uz = -fabsf(rng.u() * 2.0f - 1.0f);
     {
       const float rr = sqrtf(1.0f - p.uz * p.uz);
-      const float th = rng.u() * kTwoPi;
       float sn, cs;
-      fd2_sincos(th, &sn, &cs);
+      fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
       p.ux = rr * cs;
       p.uy = rr * sn;
     }
@@ -692,9 +687,8 @@ __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
     p.nz = -1.0f;
     p.uz = -fabsf(rng.u() * 2.0f - 1.0f);
     const float rr = sqrtf(1.0f - p.uz * p.uz);
-    const float th = rng.u() * kTwoPi;
     float sn, cs;
-    fd2_sincos(th, &sn, &cs);
+    fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
     p.ux = rr * cs;
     p.uy = rr * sn;
     return kBaAlive;
@@ -755,7 +749,7 @@ prpt:
   }
   if (inside) {
     const float ax = p.xc - p.px, ay = p.yc - p.py;
-    const float inv = 1.0f / sqrtf(ax * ax + ay * ay);
+    const float inv = fd2_rsqrt(ax * ax + ay * ay);
     p.nx = ax * inv;
     p.ny = ay * inv;
     p.nz = 0.0f;

@@ -10,6 +10,12 @@
  *               reduction by pi/2 in three parts, then the Taylor-based
  *               kernels sin r = r + r^3 P(r^2), cos r = 1 - r^2/2 + r^4 Q(r^2)
  *               on |r| <= pi/4;
+ *   fd2_sincos_turns  sin and cos of 2 pi u (an azimuth drawn as a uniform
+ *               u): the quarter-turn count 4u - rint(4u) is exact, so the
+ *               reduction is one product by pi/2, then the same kernels;
+ *   fd2_rsqrt   1/sqrt(x), x > 0 normal: the bit-level estimate
+ *               0x5f375a86 - (bits(x) >> 1) (C. Lomont, "Fast inverse square
+ *               root", 2003) and three Newton steps y (3/2 - (x/2) y^2);
  *   fd2_log     x > 0 normal; x = 2^e m, m in [sqrt(1/2), sqrt(2)),
  *               log m = 2s + s z L(z) with s = (m-1)/(m+1), z = s^2;
  *   fd2_asin / fd2_acos  |x| <= 1 (NaN outside), asin x = x + x z A(z) on
@@ -54,20 +60,9 @@ FD2_HD float fd2_rint(float x) {
   return (x + big) - big;
 }
 
-FD2_HD void fd2_sincos(float x, float* s, float* c) {
-  if (!(fabsf(x) <= 65536.0f)) { /* outside the reduction's range, inf, NaN */
-    *s = NAN;
-    *c = NAN;
-    return;
-  }
-  const float two_over_pi = 0.63661975f;
-  const float pio2_1 = 1.57079637f;      /* float(pi/2)            */
-  const float pio2_2 = -4.37113883e-08f; /* pi/2 - pio2_1          */
-  const float pio2_3 = -1.77635684e-15f; /* remainder              */
-  const float k = fd2_rint(x * two_over_pi);
-  float r = fmaf(-k, pio2_1, x);
-  r = fmaf(-k, pio2_2, r);
-  r = fmaf(-k, pio2_3, r);
+/* sin and cos of r, |r| <= pi/4, rotated by k quarter turns (k an exact
+ * small integer) */
+FD2_HD void fd2_sincos_quadrant(float r, float k, float* s, float* c) {
   const float z = r * r;
   float ps = fmaf(z, 2.724937303e-06f, -1.984008122e-04f);
   ps = fmaf(z, ps, 8.333331905e-03f);
@@ -86,6 +81,39 @@ FD2_HD void fd2_sincos(float x, float* s, float* c) {
   else { ss = -cr; cc = sr; }
   *s = ss;
   *c = cc;
+}
+
+FD2_HD void fd2_sincos(float x, float* s, float* c) {
+  if (!(fabsf(x) <= 65536.0f)) { /* outside the reduction's range, inf, NaN */
+    *s = NAN;
+    *c = NAN;
+    return;
+  }
+  const float two_over_pi = 0.63661975f;
+  const float pio2_1 = 1.57079637f;      /* float(pi/2)            */
+  const float pio2_2 = -4.37113883e-08f; /* pi/2 - pio2_1          */
+  const float pio2_3 = -1.77635684e-15f; /* remainder              */
+  const float k = fd2_rint(x * two_over_pi);
+  float r = fmaf(-k, pio2_1, x);
+  r = fmaf(-k, pio2_2, r);
+  r = fmaf(-k, pio2_3, r);
+  fd2_sincos_quadrant(r, k, s, c);
+}
+
+/* sin(2 pi u), cos(2 pi u) for |u| <= 2^20 */
+FD2_HD void fd2_sincos_turns(float u, float* s, float* c) {
+  const float t = 4.0f * u;
+  const float k = fd2_rint(t);
+  fd2_sincos_quadrant((t - k) * 1.57079637f, k, s, c);
+}
+
+FD2_HD float fd2_rsqrt(float x) {
+  float y = fd2_bits_to_float(0x5f375a86u - (fd2_float_to_bits(x) >> 1));
+  const float hx = 0.5f * x;
+  y = y * fmaf(-hx * y, y, 1.5f);
+  y = y * fmaf(-hx * y, y, 1.5f);
+  y = y * fmaf(-hx * y, y, 1.5f);
+  return y;
 }
 
 FD2_HD float fd2_sin(float x) {

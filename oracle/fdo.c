@@ -24,8 +24,6 @@
 #define FDO_TWOPI 6.283185308
 #define FDO_PI 3.14159265
 #define FDO_EPS 8.1929093e-6
-#define FDO_TWOPI_F 6.283185308f
-#define FDO_PI_F 3.14159265f
 /* Safety cap on one photon's loop passes (in-column steps plus the
  * boundary_analysis / top-surface loops the reference leaves unbounded),
  * identical to the GPU kernel's: past it the photon is ended as lost. */
@@ -240,13 +238,12 @@ static void rough(ctx *c, float *n, float beta) {
   float rr = (float)sqrt((double)(1.0f - s * s));
   s = uni(c);
   float p0, p1, nb;
-  if (c->dev) {
-    float th = s * 2.0f * FDO_PI_F;
+  if (c->dev) { /* kernel: azimuth in turns, reciprocal square root */
     float sn, cs;
-    fd2_sincos(th, &sn, &cs);
+    fd2_sincos_turns(s, &sn, &cs);
     p0 = rr * cs;
     p1 = rr * sn;
-    nb = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+    nb = 0.0f;
   } else {
     float th = (float)((double)(s * 2.0f) * FDO_PI);
     p0 = (float)((double)rr * cos((double)th));
@@ -254,7 +251,7 @@ static void rough(ctx *c, float *n, float beta) {
     nb = (float)sqrt(pow(p0, 2) + pow(p1, 2) + pow(p2, 2));
   }
   if (c->dev) {
-    const float inv = 1.0f / nb;
+    const float inv = fd2_rsqrt(p0 * p0 + p1 * p1 + p2 * p2);
     p0 = p0 * inv;
     p1 = p1 * inv;
     p2 = p2 * inv;
@@ -267,7 +264,7 @@ static void rough(ctx *c, float *n, float beta) {
   float r1 = beta * p1 + n[1];
   float r2 = beta * p2 + n[2];
   if (c->dev) {
-    const float inv = 1.0f / sqrtf(r0 * r0 + r1 * r1 + r2 * r2);
+    const float inv = fd2_rsqrt(r0 * r0 + r1 * r1 + r2 * r2);
     n[0] = r0 * inv;
     n[1] = r1 * inv;
     n[2] = r2 * inv;
@@ -282,10 +279,9 @@ static void rough(ctx *c, float *n, float beta) {
 /* x,y direction cosines rr*(cos th, sin th) with th = u*2pi: double in the
  * C path, FP32 (fd2_math.h) in device mode */
 static void azimuth(ctx *c, float rr, float u, float *dx, float *dy) {
-  if (c->dev) {
-    float th = u * FDO_TWOPI_F;
+  if (c->dev) { /* kernel: azimuth in turns */
     float sn, cs;
-    fd2_sincos(th, &sn, &cs);
+    fd2_sincos_turns(u, &sn, &cs);
     *dx = rr * cs;
     *dy = rr * sn;
   } else {
@@ -500,7 +496,7 @@ static int column_normal(ctx *c, const float *md, int *tries, int always_rough) 
     float den = (float)sqrt((double)(n[0] * n[0] + n[1] * n[1]));
     n[2] = 0.0f;
     if (c->dev) {
-      const float inv = 1.0f / den;
+      const float inv = fd2_rsqrt(n[0] * n[0] + n[1] * n[1]);
       n[0] = n[0] * inv;
       n[1] = n[1] * inv;
     } else {
@@ -792,9 +788,10 @@ prpt:  /* K:1679 */
     float mag = (float)sqrt((double)(((c->xc) - p[0]) * ((c->xc) - p[0]) +
                                      ((c->yc) - p[1]) * ((c->yc) - p[1])));
     if (c->dev) {
-      const float inv = 1.0f / mag;
-      n[0] = ((c->xc) - p[0]) * inv;
-      n[1] = ((c->yc) - p[1]) * inv;
+      const float ax = (c->xc) - p[0], ay = (c->yc) - p[1];
+      const float inv = fd2_rsqrt(ax * ax + ay * ay);
+      n[0] = ax * inv;
+      n[1] = ay * inv;
     } else {
       n[0] = ((c->xc) - p[0]) / mag;
       n[1] = ((c->yc) - p[1]) / mag;
@@ -1114,6 +1111,9 @@ int fdo_math_eval(int fn, const float *x, float *y, int64_t n) {
       case 2: y[i] = fd2_log(x[i]); break;
       case 3: y[i] = fd2_asin(x[i]); break;
       case 4: y[i] = fd2_acos(x[i]); break;
+      case 5: { float sn, cs; fd2_sincos_turns(x[i], &sn, &cs); y[i] = sn; break; }
+      case 6: { float sn, cs; fd2_sincos_turns(x[i], &sn, &cs); y[i] = cs; break; }
+      case 7: y[i] = fd2_rsqrt(x[i]); break;
       default: return -1;
     }
   }

@@ -248,7 +248,8 @@ def counter_stream(seed: int, event: int, photon: int, n: int) -> np.ndarray:
     return out
 
 
-MATH_FN = {"sin": 0, "cos": 1, "log": 2, "asin": 3, "acos": 4}
+MATH_FN = {"sin": 0, "cos": 1, "log": 2, "asin": 3, "acos": 4,
+           "sin_turns": 5, "cos_turns": 6, "rsqrt": 7}
 
 
 def math_eval(fn: str, x) -> np.ndarray:

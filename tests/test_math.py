@@ -15,6 +15,9 @@ def ulp_err(got, ref):
     ("log", 6e-8, 1.0, np.log, 2.0),
     ("asin", -1.0, 1.0, np.arcsin, 3.0),
     ("acos", -1.0, 1.0, np.arccos, 2.0),
+    ("sin_turns", 0.0, 1.0, lambda u: np.sin(2 * np.pi * u), 2.0),
+    ("cos_turns", 0.0, 1.0, lambda u: np.cos(2 * np.pi * u), 2.0),
+    ("rsqrt", 1e-6, 1e6, lambda x: 1.0 / np.sqrt(x), 2.0),
 ])
 def test_accuracy(oracle, fn, lo, hi, ref, tol):
     x = np.linspace(lo, hi, 400001).astype(np.float32)
@@ -32,3 +35,21 @@ def test_domains(oracle):
     y = oracle.math_eval("acos", [-1.0])[0]
     assert abs(y - np.pi) < 3e-7
 
+
+
+def test_turns_and_rsqrt_special_values(oracle):
+    """Exact quarter turns give exact zeros and ones; the uniforms the
+    kernel draws (24-bit, in (0, 1]) stay within 2 ulp; rsqrt over many
+    binades including values just below 1 (normalisations)."""
+    u = np.array([0.25, 0.5, 0.75, 1.0], np.float32)
+    assert np.array_equal(oracle.math_eval("sin_turns", u), [1.0, 0.0, -1.0, 0.0])
+    assert np.array_equal(oracle.math_eval("cos_turns", u), [0.0, -1.0, 0.0, 1.0])
+    k = np.arange(1, 1 << 16, dtype=np.float64)
+    uu = (k * 256.0 * 2.0 ** -24).astype(np.float32)  # a stride of the draws' lattice
+    for fn, ref in (("sin_turns", np.sin), ("cos_turns", np.cos)):
+        r = ref(2 * np.pi * uu.astype(np.float64))
+        keep = np.abs(r) > 1e-3
+        assert ulp_err(oracle.math_eval(fn, uu)[keep], r[keep]).max() <= 2.0
+    x = np.concatenate([np.geomspace(1e-30, 1e30, 20001),
+                        np.linspace(0.999, 1.001, 20001)]).astype(np.float32)
+    assert ulp_err(oracle.math_eval("rsqrt", x), 1.0 / np.sqrt(x.astype(np.float64))).max() <= 2.0
