@@ -366,7 +366,6 @@ __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, 
     parked = false;
   }
   n_back += m;
-  __threadfence_block();
   return true;
 }
 
@@ -432,6 +431,7 @@ k_transport(TransportArgs a) {
     unsigned long long need = __ballot(!alive);
     if (__popcll(need) < a.refill_lanes && need != ~0ull) need = 0ull;
     if (QUEUE && need != 0ull && n_back != 0u) {
+      __threadfence_block();  // the wave's queue stores are complete before its loads
       const uint32_t take = min((uint32_t)__popcll(need), n_back);
       const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
       if (!alive && rank < take) {
@@ -520,7 +520,6 @@ k_transport(TransportArgs a) {
           parked = false;
         }
         n_wall += k;
-        __threadfence_block();
       }
       // With every lane idle and no new work, the queues drain: wall first.
       const bool lanes_idle = __ballot(alive) == 0ull && pool_done && n_back == 0u;
@@ -531,6 +530,7 @@ k_transport(TransportArgs a) {
           break;
         }
         const uint32_t take = min(n_wall, 64u);
+        __threadfence_block();
         if ((uint32_t)lane < take) {
           bool pk;
           q_get(wq, n_wall - take + (uint32_t)lane, p, rng, lane_ev, gi, lane_k, pk);
@@ -557,7 +557,6 @@ k_transport(TransportArgs a) {
             alive = false;
           }
           n_gap += (uint32_t)__popcll(gm);
-          __threadfence_block();
         }
       }
       if (n_gap >= 64u || (n_gap != 0u && n_wall == 0u && __ballot(alive) == 0ull && pool_done &&
@@ -569,6 +568,7 @@ k_transport(TransportArgs a) {
         }
         const uint32_t take = min(n_gap, 64u);
         PrptState st{0u, 0.0f, 0.0f, 0.0f};
+        __threadfence_block();
         if ((uint32_t)lane < take) {
           bool pk;
           const uint32_t slot = kGapBase + n_gap - take + (uint32_t)lane;
