@@ -718,10 +718,11 @@ __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
 // The reference runs this as one goto loop whose locals persist across its
 // passes: the resample/retry counters, the stale Pr/Pt of the gap side
 // without Fresnel recomputation, and the stale theta2 that a refraction
-// then uses.  PrptState holds them so the loop can stop after the
-// refraction out of a column (kPrGap) and continue later with the flight
-// through the gap (inside = false); the photon's arithmetic and draws are
-// the same whether or not it stops there.
+// then uses.  PrptState holds them so that, with stop_at_gap, the loop stops
+// before each flight through the gap (after the refraction out of a column,
+// and after a reflection off the outside of the next column) and returns
+// kPrGap; a later call with inside = false continues with that flight.  The
+// photon's arithmetic and draws are the same whether or not it stops there.
 struct PrptState {
   uint32_t tries;  // theta, smooth, reperturb, normal counters, 8 bits each
   float pr, pt, t2;
@@ -833,27 +834,23 @@ no_perturbation:
       p.fate = kTheta1;
       return kPrEnded;
     }
-    if (!inside) {
-      if (transmit_gap(rng, p, g)) return kPrEnded;
-      goto prpt;
-    }
+    if (!inside) goto gap_flight;
     return kPrAlive;
   }
   refract(p, t1, t2, g);
-  if (inside) {
-    inside = false;
-    if (stop_at_gap) {
-      st.tries = (uint32_t)theta_tries | ((uint32_t)smooth_tries << 8) |
-                 ((uint32_t)reperturb << 16) | ((uint32_t)normal_tries << 24);
-      st.pr = pr;
-      st.pt = pt;
-      st.t2 = t2;
-      return kPrGap;
-    }
-    if (transmit_gap(rng, p, g)) return kPrEnded;
-    goto prpt;
+  if (!inside) return kPrAlive;
+  inside = false;
+gap_flight:
+  if (stop_at_gap) {
+    st.tries = (uint32_t)theta_tries | ((uint32_t)smooth_tries << 8) |
+               ((uint32_t)reperturb << 16) | ((uint32_t)normal_tries << 24);
+    st.pr = pr;
+    st.pt = pt;
+    st.t2 = t2;
+    return kPrGap;
   }
-  return kPrAlive;
+  if (transmit_gap(rng, p, g)) return kPrEnded;
+  goto prpt;
 }
 
 // The whole branch in one call.  true = ended.

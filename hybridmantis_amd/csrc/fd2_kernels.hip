@@ -376,7 +376,9 @@ __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, 
 // wave moves its photons in flight to its backlog and runs the branch with
 // all 64 lanes, up to a refraction out of the column; those photons wait in
 // the gap queue and their flights through the gap run, again 64 at a time,
-// in a phase of their own.  Free lanes refill from the backlog before the
+// one flight per photon and pass, in a phase of their own (a photon
+// reflected off the next column's outside waits for another flight).  Free
+// lanes refill from the backlog before the
 // work queue, so no more than 64 + kWallCap + kGapCap photons are ever in
 // flight per wave and the backlog (kBackCap) cannot overflow; the gap queue
 // holds at most 63 + 64.  Every photon keeps its own state and random
@@ -559,11 +561,17 @@ k_transport(TransportArgs a) {
           n_gap += (uint32_t)__popcll(gm);
         }
       }
-      if (n_gap >= 64u || (n_gap != 0u && n_wall == 0u && __ballot(alive) == 0ull && pool_done &&
-                           n_back == 0u)) {
+      // Gap flights: a pass takes up to 64 waiting photons and runs one
+      // flight each (to the next column and its Fresnel decision); a photon
+      // reflected off the outside of that column goes back to the gap queue
+      // for its next flight.  Passes repeat while 64 wait, so the queue
+      // holds at most 63 entries after this loop and 127 during it.
+      bool failed = false;
+      while (n_gap >= 64u || (n_gap != 0u && n_wall == 0u && __ballot(alive) == 0ull &&
+                              pool_done && n_back == 0u)) {
         progress = true;
         if (!swap_to_backlog(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k, lt_mask)) {
-          if (lane == 0) atomicOr(a.error_flag, 2u);
+          failed = true;
           break;
         }
         const uint32_t take = min(n_gap, 64u);
@@ -577,14 +585,28 @@ k_transport(TransportArgs a) {
           alive = true;
         }
         n_gap -= take;
-        if (alive) {
-          const int r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, false, false);
-          if (r == kPrEnded) {
-            finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
-                                              tile_area, run_hist, run_g, run_cnt);
+        int r = kPrAlive;
+        if (alive) r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, false, true);
+        if (alive && r == kPrEnded) {
+          finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+                                            tile_area, run_hist, run_g, run_cnt);
+          alive = false;
+        }
+        // the slots just read are free again: [n_gap, n_gap + take)
+        const unsigned long long gm = __ballot(alive && r == kPrGap);
+        if (gm != 0ull) {
+          if (alive && r == kPrGap) {
+            const uint32_t slot = kGapBase + n_gap + (uint32_t)__popcll(gm & lt_mask);
+            q_put(wq, slot, p, rng, lane_ev, gi, lane_k, false);
+            q_put_state(wq, slot, st);
             alive = false;
           }
+          n_gap += (uint32_t)__popcll(gm);
         }
+      }
+      if (failed) {
+        if (lane == 0) atomicOr(a.error_flag, 2u);
+        break;
       }
       if (progress) {
         idle = 0;
