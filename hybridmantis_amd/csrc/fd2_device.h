@@ -144,7 +144,9 @@ __device__ __forceinline__ void ranecu_init(RanecuRng& g, int batch,
 enum Fate : int {
   kNone = 0, kDetect = 1, kDetectOut = 2, kAbsTop = 3, kAbsBulk = 4,
   kLost = 5, kOutOfCol = 6, kTheta1 = 7,
-  kResume = 8  // alive: reflected by the sensor, its next in-column step is due
+  kResume = 8,     // alive: reflected by the sensor, its next in-column step is due
+  kNewColumn = 9   // alive: a crosstalk's neighbour-column sample is due
+                   // (pix holds the samples rejected so far)
 };
 
 // photon state, all in registers.  The event's position and history stay
@@ -467,35 +469,41 @@ __device__ __forceinline__ int detection(Rng& rng, Photon& p, const GeomDev& g) 
 // main/kernel_cuda_c_ver1_0.cu:2452-2488, 2501-2538, 1769-1804, 3036-3070):
 // n = -d perturbed with beta = 1, projected on xy and renormalised, then
 // perturbed with beta (always for refl_bottom); accept when
-// acos(-d.n) <= 1.57, else resample while the caller's counter < 100.
-// true = photon killed (theta1).
+// acos(-d.n) <= 1.57.  One sample; true = accepted (p.n is the sample
+// either way).
+template <class Rng>
+__device__ __forceinline__ bool column_normal_sample(Rng& rng, Photon& p, const GeomDev& g,
+                                                     float mx, float my, float mz,
+                                                     bool always_rough) {
+  float nx = mx, ny = my, nz = mz;
+  rough(rng, nx, ny, nz, 1.0f);
+  const float inv = fd2_rsqrt(nx * nx + ny * ny);
+  nz = 0.0f;
+  nx = nx * inv;
+  ny = ny * inv;
+  if (always_rough || g.beta > 0.0f) rough(rng, nx, ny, nz, g.beta);
+  p.nx = nx;
+  p.ny = ny;
+  p.nz = nz;
+  const float ang = fd2_acos(mx * nx + my * ny + mz * nz);
+  return !(ang < 0.0f || ang > 1.57f);
+}
+
+// Samples until accepted, while the caller's counter < 100.  true = photon
+// killed (theta1).
 template <class Rng>
 __device__ __forceinline__ bool column_normal(Rng& rng, Photon& p,
                                               const GeomDev& g, float mx,
                                               float my, float mz, int& tries,
                                               bool always_rough) {
-  for (;;) {
-    float nx = mx, ny = my, nz = mz;
-    rough(rng, nx, ny, nz, 1.0f);
-    const float inv = fd2_rsqrt(nx * nx + ny * ny);
-    nz = 0.0f;
-    nx = nx * inv;
-    ny = ny * inv;
-    if (always_rough || g.beta > 0.0f) rough(rng, nx, ny, nz, g.beta);
-    p.nx = nx;
-    p.ny = ny;
-    p.nz = nz;
-    const float ang = fd2_acos(mx * nx + my * ny + mz * nz);
-    if (ang < 0.0f || ang > 1.57f) {
-      if (tries < 100) {
-        tries++;
-        continue;
-      }
+  while (!column_normal_sample(rng, p, g, mx, my, mz, always_rough)) {
+    if (tries >= 100) {
       p.fate = kTheta1;
       return true;
     }
-    return false;
+    tries++;
   }
+  return false;
 }
 
 __device__ __forceinline__ bool at_top_up(const Photon& p, const GeomDev& g) {
@@ -665,17 +673,19 @@ __device__ __forceinline__ void fresnel_pq(float t1, float t2, float& pr,
   pt = 0.5f * (s12 / (sp2 * (cm * cm)) + s12 / sp2);
 }
 
-// boundary_analysis (main/kernel_cuda_c_ver1_0.cu:1582-1958), split in two.
+// boundary_analysis (main/kernel_cuda_c_ver1_0.cu:1582-1958), split up.
 // ba_entry is the frequent part -- the top reflector and the crosstalk
-// decision with its new-column normal (K:1610-1673); it returns kBaAlive,
-// kBaEnded or kBaPrpt when the wall's Fresnel branch must follow (K:1677-1955,
-// ba_prpt below).  Running ba_prpt later for a lane changes nothing for
-// that photon: all its state and its random stream go with it.
-enum BaResult : int { kBaAlive = 0, kBaEnded = 1, kBaPrpt = 2 };
+// decision (K:1610-1652); it returns kBaAlive, kBaEnded, kBaCross when
+// crosstalk was drawn and the neighbour column's orientation follows
+// (K:1655-1673, sampled by hot_step), or kBaPrpt when the wall's Fresnel
+// branch must follow (K:1677-1955, ba_prpt below).  Running either later
+// for a lane changes nothing for that photon: all its state and its random
+// stream go with it.
+enum BaResult : int { kBaAlive = 0, kBaEnded = 1, kBaPrpt = 2, kBaCross = 3 };
 
 template <class Rng>
 __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
-  const float H2 = g.half_h, R = g.R;
+  const float H2 = g.half_h;
   // top reflector reached inside a column (K:1610-1634)
   if (fabsf(p.pz - H2) < g.eps_lt && p.uz > 0.0f) {
     if (g.top_absfrac > 0.0f && rng.u() < g.top_absfrac) {
@@ -701,14 +711,7 @@ __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
     cct = p.pz / (2.0f * (H2 - g.depth20)) + 0.5f;
   else if (p.pz < 0.0f && p.pz >= -H2)
     cct = (p.pz - (-H2)) / (2.0f * (-H2)) + 1.0f;
-  if (rng.u() < cct) {
-    // neighbour column with a random orientation, direction kept (K:2452-2488)
-    int tries = 0;
-    if (column_normal(rng, p, g, -p.ux, -p.uy, -p.uz, tries, false)) return kBaEnded;
-    p.xc = p.px + R * (-p.nx);
-    p.yc = p.py + R * (-p.ny);
-    return kBaAlive;
-  }
+  if (rng.u() < cct) return kBaCross;
   return kBaPrpt;
 }
 
@@ -864,34 +867,61 @@ __device__ __forceinline__ bool ba_prpt(Rng& rng, Photon& p, const GeomDev& g) {
 // fused with the boundary interaction that follows it: in-column step and
 // sensor check, then (if alive and not trapped) boundary_analysis up to its
 // Fresnel branch; kNeedPrpt asks the caller to run ba_prpt.
+//
+// Crosstalk moves the photon to a neighbour column whose orientation is
+// sampled from -d until a sample passes the angle test, at most 101 samples
+// (K:2452-2488).  A pass takes one sample: a rejected photon is left in
+// state kNewColumn and its next pass samples again instead of stepping, so
+// its draws and arithmetic are the same, and lanes that need one sample do
+// not wait in a loop for lanes that need several.
 enum StepResult : int { kAlive = 0, kEnded = 1, kTrapped = 2, kNeedPrpt = 3 };
 
 template <class Rng, bool TRACE>
 __device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
                                         StepSink* sink) {
-  // a pass resumed after a sensor reflection is the rest of the previous
-  // pass: no loop count, no second sensor check
-  const bool resumed = p.fate == kResume;
-  if (resumed) {
-    p.fate = kNone;
-  } else if (++p.loops > kMaxLoops) {
-    p.fate = kLost;
+  if (p.fate != kNewColumn) {
+    // a pass resumed after a sensor reflection is the rest of the previous
+    // pass: no loop count, no second sensor check
+    const bool resumed = p.fate == kResume;
+    if (resumed) {
+      p.fate = kNone;
+    } else if (++p.loops > kMaxLoops) {
+      p.fate = kLost;
+      return kEnded;
+    }
+    if (isotropic<Rng, TRACE>(rng, p, g, sink)) return kEnded;
+    if (!resumed) {
+      const int d = detection(rng, p, g);
+      if (d == kSensorDetect) return kEnded;
+      if (d == kSensorReflect) {
+        p.fate = kResume;
+        return kAlive;
+      }
+    }
+    if (fabsf(p.uz) < g.eps_lt) return kTrapped;
+    p.first = false;
+    const int b = ba_entry(rng, p, g);
+    if (b == kBaEnded) return kEnded;
+    if (b == kBaPrpt) return kNeedPrpt;
+    if (b == kBaAlive) return kAlive;
+    p.pix = 0;  // crosstalk: no sample rejected yet
+  }
+  // neighbour column with a random orientation, direction kept
+  const int rejected = p.pix;
+  p.pix = -1;
+  p.fate = kNone;
+  if (column_normal_sample(rng, p, g, -p.ux, -p.uy, -p.uz, false)) {
+    p.xc = p.px + g.R * (-p.nx);
+    p.yc = p.py + g.R * (-p.ny);
+    return kAlive;
+  }
+  if (rejected >= 100) {
+    p.fate = kTheta1;
     return kEnded;
   }
-  if (isotropic<Rng, TRACE>(rng, p, g, sink)) return kEnded;
-  if (!resumed) {
-    const int d = detection(rng, p, g);
-    if (d == kSensorDetect) return kEnded;
-    if (d == kSensorReflect) {
-      p.fate = kResume;
-      return kAlive;
-    }
-  }
-  if (fabsf(p.uz) < g.eps_lt) return kTrapped;
-  p.first = false;
-  const int b = ba_entry(rng, p, g);
-  if (b == kBaEnded) return kEnded;
-  return b == kBaPrpt ? kNeedPrpt : kAlive;
+  p.fate = kNewColumn;
+  p.pix = rejected + 1;
+  return kAlive;
 }
 
 // the whole pass, for kernels that do not defer the Fresnel branch

@@ -292,8 +292,11 @@ __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon&
   w[8 * kQSlots] = __float_as_uint(p.nz);
   w[9 * kQSlots] = __float_as_uint(p.xc);
   w[10 * kQSlots] = __float_as_uint(p.yc);
+  // loops (21 bits) | first | sensor resume | parked | new-column sample due,
+  // with its rejected samples (<= 100) in bits 25-31
   w[11 * kQSlots] = (uint32_t)p.loops | (p.first ? 1u << 21 : 0u) |
-                    (p.fate == kResume ? 1u << 22 : 0u) | (parked ? 1u << 23 : 0u);
+                    (p.fate == kResume ? 1u << 22 : 0u) | (parked ? 1u << 23 : 0u) |
+                    (p.fate == kNewColumn ? (1u << 24) | ((uint32_t)p.pix << 25) : 0u);
   w[12 * kQSlots] = r.s0;
   w[13 * kQSlots] = r.s1;
   w[14 * kQSlots] = r.s2;
@@ -336,9 +339,9 @@ __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon&
   const uint32_t f = w[11 * kQSlots];
   p.loops = (int32_t)(f & 0x1fffffu);
   p.first = (f >> 21) & 1u;
-  p.fate = ((f >> 22) & 1u) ? kResume : kNone;
+  p.fate = ((f >> 22) & 1u) ? kResume : ((f >> 24) & 1u) ? kNewColumn : kNone;
   parked = (f >> 23) & 1u;
-  p.pix = -1;
+  p.pix = ((f >> 24) & 1u) ? (int32_t)(f >> 25) : -1;
   r.s0 = w[12 * kQSlots];
   r.s1 = w[13 * kQSlots];
   r.s2 = w[14 * kQSlots];
