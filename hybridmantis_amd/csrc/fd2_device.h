@@ -324,40 +324,22 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
     dcyl = r1;
   else if (r2 > r1)
     dcyl = r2;
-  float d;
-  int plane = 0;  // traced: the plane z = +-H/2 was the nearer surface
-  if (p.uz > 0.0f) {
-    const float dpl = (H2 - p.pz) / p.uz;
-    if (fabsf(p.ux) < g.eps_lt && fabsf(p.uy) < g.eps_lt && fabsf(p.uz - 1.0f) < g.eps_lt) {
+  // the plane z = +H/2 (going up) or -H/2 (going down) when it is nearer
+  // than the wall, or when the direction is along the axis
+  float d = dcyl;
+  int plane = 0;  // traced: the plane was the nearer surface
+  const bool up = p.uz > 0.0f;
+  const float zpl = up ? H2 : -H2;
+  if (up || p.uz < 0.0f) {
+    const float dpl = (zpl - p.pz) / p.uz;
+    const bool axial = fabsf(p.ux) < g.eps_lt && fabsf(p.uy) < g.eps_lt &&
+                       fabsf(p.uz - (up ? 1.0f : -1.0f)) < g.eps_lt;
+    if (axial || !(dpl >= dcyl)) {
       d = dpl;
-      p.pz = H2;
-      plane = 1;
-    } else if (dpl >= dcyl) {
-      d = dcyl;
-      p.pz = t2 + d * p.uz;
-    } else {
-      d = dpl;
-      p.pz = H2;
       plane = 1;
     }
-  } else if (p.uz < 0.0f) {
-    const float dpl = (-H2 - p.pz) / p.uz;
-    if (fabsf(p.ux) < g.eps_lt && fabsf(p.uy) < g.eps_lt && fabsf(p.uz + 1.0f) < g.eps_lt) {
-      d = dpl;
-      p.pz = -H2;
-      plane = 1;
-    } else if (dpl >= dcyl) {
-      d = dcyl;
-      p.pz = t2 + d * p.uz;
-    } else {
-      d = dpl;
-      p.pz = -H2;
-      plane = 1;
-    }
-  } else {
-    d = dcyl;
-    p.pz = t2 + d * p.uz;
   }
+  p.pz = plane ? zpl : t2 + d * p.uz;
   p.px = t0 + d * p.ux;
   p.py = t1 + d * p.uy;
 
@@ -704,13 +686,19 @@ __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
     return kBaAlive;
   }
   // crosstalk probability, piecewise linear in depth (K:1639-1652)
+  const float top20 = H2 - g.depth20;
   float cct = 0.0f;
-  if (p.pz <= H2 && p.pz >= H2 - g.depth20)
+  if (p.pz <= H2 && p.pz >= top20) {
     cct = 1.0f;
-  else if (p.pz < H2 - g.depth20 && p.pz >= 0.0f)
-    cct = p.pz / (2.0f * (H2 - g.depth20)) + 0.5f;
-  else if (p.pz < 0.0f && p.pz >= -H2)
-    cct = (p.pz - (-H2)) / (2.0f * (-H2)) + 1.0f;
+  } else {
+    // one division for both linear pieces (z in [0, top20) or [-H/2, 0))
+    const bool upper = p.pz < top20 && p.pz >= 0.0f;
+    if (upper || (p.pz < 0.0f && p.pz >= -H2)) {
+      const float num = upper ? p.pz : p.pz - (-H2);
+      const float den = upper ? 2.0f * top20 : 2.0f * (-H2);
+      cct = num / den + (upper ? 0.5f : 1.0f);
+    }
+  }
   if (rng.u() < cct) return kBaCross;
   return kBaPrpt;
 }
