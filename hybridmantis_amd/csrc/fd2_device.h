@@ -199,7 +199,7 @@ __device__ __forceinline__ void tally_detect(Photon& p, const GeomDev& g) {
 
 __device__ __forceinline__ void renorm(float& x, float& y, float& z,
                                        const GeomDev& g) {
-  float nrm = sqrtf(x * x + y * y + z * z);
+  float nrm = fd2_sqrt_unit(x * x + y * y + z * z);
   if (nrm < g.norm_lo || nrm > g.norm_hi) {
     const float inv = 1.0f / nrm;
     x = x * inv;
@@ -237,7 +237,7 @@ __device__ __forceinline__ void rough(Rng& rng, float& nx, float& ny, float& nz,
                                       float beta) {
   float s = rng.u();
   float qz = 2.0f * s - 1.0f;
-  float rr = sqrtf(1.0f - s * s);
+  float rr = fd2_sqrt_unit(1.0f - s * s);
   s = rng.u();
   float sn, cs;
   fd2_sincos_turns(s, &sn, &cs);  // azimuth 2 pi s
@@ -259,7 +259,7 @@ template <class Rng>
 __device__ __forceinline__ void emit_dir(Rng& rng, Photon& p, const GeomDev& g) {
   float r = rng.u() * 2.0f - 1.0f;
   p.uz = r;
-  float rr = sqrtf(1.0f - r * r);
+  float rr = fd2_sqrt_unit(1.0f - r * r);
   float sn, cs;
   fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
   p.ux = rr * cs;
@@ -275,7 +275,7 @@ __device__ __forceinline__ void emit_first_dir(Rng& rng, Photon& p,
   float r = rng.u() * 2.0f - 1.0f;
   while (fabsf(r) <= 0.01f) r = rng.u() * 2.0f - 1.0f;
   p.uz = r;
-  float rr = sqrtf(1.0f - r * r);
+  float rr = fd2_sqrt_unit(1.0f - r * r);
   float sn, cs;
   fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
   p.ux = rr * cs;
@@ -447,6 +447,16 @@ __device__ __forceinline__ int detection(Rng& rng, Photon& p, const GeomDev& g) 
   return kSensorNone;
 }
 
+// fd2_acos(x) < 0 || fd2_acos(x) > 1.57 without branches.  The arc cosine
+// crosses 1.57 at x = cos(1.57) = 7.9633e-4; over all floats x in [-1, 1],
+// fd2_acos(x) >= 0, x <= 7.90e-4 gives an angle above 1.57 and x >= 8.03e-4
+// one at or below it, and between the two fd2_acos is its |x| <= 1/2 branch
+// (fd2_acos_small).  |x| > 1 and NaN give a NaN angle, which is neither.
+__device__ __forceinline__ bool angle_over_157(float x) {
+  const float near = fd2_acos_small(x);
+  return (x >= -1.0f && x <= 7.90e-4f) || (x > 7.90e-4f && x < 8.03e-4f && near > 1.57f);
+}
+
 // New column orientation (transmit/boundary_analysis/refl_bottom,
 // main/kernel_cuda_c_ver1_0.cu:2452-2488, 2501-2538, 1769-1804, 3036-3070):
 // n = -d perturbed with beta = 1, projected on xy and renormalised, then
@@ -467,8 +477,7 @@ __device__ __forceinline__ bool column_normal_sample(Rng& rng, Photon& p, const 
   p.nx = nx;
   p.ny = ny;
   p.nz = nz;
-  const float ang = fd2_acos(mx * nx + my * ny + mz * nz);
-  return !(ang < 0.0f || ang > 1.57f);
+  return !angle_over_157(mx * nx + my * ny + mz * nz);
 }
 
 // Samples until accepted, while the caller's counter < 100.  true = photon
@@ -598,7 +607,7 @@ top:
     }
     p.uz = -fabsf(rng.u() * 2.0f - 1.0f);
     {
-      const float rr = sqrtf(1.0f - p.uz * p.uz);
+      const float rr = fd2_sqrt_unit(1.0f - p.uz * p.uz);
       float sn, cs;
       fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
       p.ux = rr * cs;
@@ -678,7 +687,7 @@ __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
     p.ny = 0.0f;
     p.nz = -1.0f;
     p.uz = -fabsf(rng.u() * 2.0f - 1.0f);
-    const float rr = sqrtf(1.0f - p.uz * p.uz);
+    const float rr = fd2_sqrt_unit(1.0f - p.uz * p.uz);
     float sn, cs;
     fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
     p.ux = rr * cs;
@@ -810,8 +819,7 @@ no_perturbation:
 
   if (rng.u() < pr) {
     reflect(p, g);
-    const float ang = fd2_acos(onx * p.ux + ony * p.uy + onz * p.uz);
-    if (ang > 1.57f) {
+    if (angle_over_157(onx * p.ux + ony * p.uy + onz * p.uz)) {
       reperturb++;
       if (reperturb < 4) goto reperturb_point;
       p.nx = onx; p.ny = ony; p.nz = onz;

@@ -605,6 +605,25 @@ int fd2_debug_steps(fd2_ctx* c, const fd2_event* events, int64_t n, int32_t max_
   return rc;
 }
 
+int fd2_debug_math(int fn, const float* x, float* y, int64_t n, int device) {
+  if (fn < 0 || fn >= FD2_MATH_NFN || !x || !y || n < 0) return FD2_E_INVAL;
+  if (n == 0) return FD2_OK;
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0) return FD2_E_NO_DEVICE;
+  if (device < 0 || device >= n_dev) return FD2_E_INVAL;
+  FD2_HIP(hipSetDevice(device));
+  float* d = nullptr;
+  const size_t bytes = (size_t)n * sizeof(float);
+  if (hipMalloc(&d, 2 * bytes) != hipSuccess) return FD2_E_NOMEM;
+  int rc = FD2_OK;
+  if (hipMemcpy(d, x, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      launch_debug_math(fn, d, d + n, n, nullptr) != hipSuccess ||
+      hipMemcpy(y, d + n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = FD2_E_DEVICE;
+  (void)hipFree(d);
+  return rc;
+}
+
 int fd2_prf_dims(const fd2_ctx* c, int geom, int32_t* xdim, int32_t* ydim) {
   if (!c || geom < 0 || geom >= c->n_geoms || !xdim || !ydim) return FD2_E_INVAL;
   *xdim = c->hgeom[geom].xdim;

@@ -84,6 +84,7 @@ hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
                               int32_t n_photons, float* steps,
                               int32_t* n_steps, int32_t max_steps,
                               hipStream_t s);
+hipError_t launch_debug_math(int fn, const float* x, float* y, int64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);

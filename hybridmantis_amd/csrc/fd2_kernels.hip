@@ -12,6 +12,7 @@
 //   k_trace_ranecu one lane per event, photons in sequence, the reference's
 //                 per-event RANECU stream (trajectory test)
 //   k_debug_steps per-step FP32 trace of a few photons (FP64 re-trace check)
+//   k_debug_math  the device arithmetic on given arguments
 #include "fd2_internal.h"
 
 namespace fd2 {
@@ -808,6 +809,38 @@ hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
   hipLaunchKernelGGL(k_debug_steps, dim3((n_photons + 63) / 64), dim3(64), 0, s,
                      events, event_base, seed, geoms, n_photons, steps, n_steps,
                      max_steps);
+  return hipGetLastError();
+}
+
+// Device arithmetic on its own: y[i] = f(x[i]) for the functions the
+// transport uses, numbered as FD2_MATH_* (the oracle evaluates the same
+// functions on the CPU, so the two can be compared bit for bit).
+__global__ void k_debug_math(int fn, const float* x, float* y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = x[i];
+  float r;
+  float sn, cs;
+  switch (fn) {
+    case FD2_MATH_SIN: r = fd2_sin(v); break;
+    case FD2_MATH_COS: r = fd2_cos(v); break;
+    case FD2_MATH_LOG: r = fd2_log(v); break;
+    case FD2_MATH_ASIN: r = fd2_asin(v); break;
+    case FD2_MATH_ACOS: r = fd2_acos(v); break;
+    case FD2_MATH_SIN_TURNS: fd2_sincos_turns(v, &sn, &cs); r = sn; break;
+    case FD2_MATH_COS_TURNS: fd2_sincos_turns(v, &sn, &cs); r = cs; break;
+    case FD2_MATH_RSQRT: r = fd2_rsqrt(v); break;
+    case FD2_MATH_SQRT_UNIT: r = fd2_sqrt_unit(v); break;
+    case FD2_MATH_ANGLE_OVER_157: r = angle_over_157(v) ? 1.0f : 0.0f; break;
+    default: r = 0.0f; break;
+  }
+  y[i] = r;
+}
+
+hipError_t launch_debug_math(int fn, const float* x, float* y, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_debug_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, x, y,
+                     n);
   return hipGetLastError();
 }
 

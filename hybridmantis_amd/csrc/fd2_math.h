@@ -18,6 +18,8 @@
  *               root", 2003) and three Newton steps y (3/2 - (x/2) y^2);
  *   fd2_log     x > 0 normal; x = 2^e m, m in [sqrt(1/2), sqrt(2)),
  *               log m = 2s + s z L(z) with s = (m-1)/(m+1), z = s^2;
+ *   fd2_sqrt_unit  sqrtf for 0 and x >= 2^-96, with fewer instructions
+ *               on the GPU (hardware root plus the correctly rounded fix-up);
  *   fd2_asin / fd2_acos  |x| <= 1 (NaN outside), asin x = x + x z A(z) on
  *               |x| <= 1/2 and the half-angle identity
  *               asin x = pi/2 - 2 asin(sqrt((1-x)/2)) above.
@@ -156,6 +158,28 @@ FD2_HD float fd2_log(float x) {
 }
 
 /* asin on [0, 1/2] */
+/* sqrtf(x) for x = +-0, x >= 2^-96, +inf or NaN (NaN for x < 0).  On the GPU:
+ * the hardware square root (within 1 ulp) moved to the correctly rounded
+ * result by the signs of its neighbours' residuals -- the compiler's own
+ * sqrtf sequence without the scaling it adds for x < 2^-96 and without its
+ * +-0/+inf pass-through, which the sequence already gets right.  On the CPU:
+ * sqrtf.  Callers: 1 - s^2 and 1 - r^2 of 24-bit uniforms (0 or >= 2^-24),
+ * (1 - |x|)/2 in asin/acos, squared norms of unit-length vectors. */
+FD2_HD float fd2_sqrt_unit(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float y = __builtin_amdgcn_sqrtf(x);
+  const float ydn = __uint_as_float(__float_as_uint(y) - 1u);
+  const float yup = __uint_as_float(__float_as_uint(y) + 1u);
+  const float rdn = fmaf(-ydn, y, x), rup = fmaf(-yup, y, x);
+  float r = y;
+  if (rdn <= 0.0f) r = ydn;
+  if (rup > 0.0f) r = yup;
+  return r;
+#else
+  return sqrtf(x);
+#endif
+}
+
 FD2_HD float fd2_asin_core(float x) {
   const float z = x * x;
   float a = fmaf(z, 3.368483111e-02f, 1.715300046e-02f);
@@ -172,19 +196,24 @@ FD2_HD float fd2_asin(float x) {
   if (ax <= 0.5f) {
     r = fd2_asin_core(ax);
   } else {
-    const float s = sqrtf((1.0f - ax) * 0.5f); /* NaN when |x| > 1 */
+    const float s = fd2_sqrt_unit((1.0f - ax) * 0.5f); /* NaN when |x| > 1 */
     const float pio2_hi = 1.57079637f, pio2_lo = -4.37113883e-08f;
     r = pio2_hi - fmaf(2.0f, fd2_asin_core(s), -pio2_lo);
   }
   return x < 0.0f ? -r : r;
 }
 
-FD2_HD float fd2_acos(float x) {
+/* acos x for |x| <= 1/2 (fd2_acos's first branch) */
+FD2_HD float fd2_acos_small(float x) {
   const float pio2_hi = 1.57079637f, pio2_lo = -4.37113883e-08f;
+  return pio2_hi - (fd2_asin_core(x) - pio2_lo);
+}
+
+FD2_HD float fd2_acos(float x) {
   const float pi_hi = 3.14159274f, pi_lo = -8.74227766e-08f;
   const float ax = fabsf(x);
-  if (ax <= 0.5f) return pio2_hi - (fd2_asin_core(x) - pio2_lo);
-  const float s = sqrtf((1.0f - ax) * 0.5f); /* NaN when |x| > 1 or NaN */
+  if (ax <= 0.5f) return fd2_acos_small(x);
+  const float s = fd2_sqrt_unit((1.0f - ax) * 0.5f); /* NaN when |x| > 1 or NaN */
   const float t = 2.0f * fd2_asin_core(s);
   if (x > 0.0f) return t;
   return pi_hi - (t - pi_lo);

@@ -97,6 +97,7 @@ def lib() -> C.CDLL:
         "fd2_trace_ranecu": ([P, P, C.c_int64, C.c_int64, C.c_int32, P], C.c_int),
         "fd2_debug_steps": ([P, P, C.c_int64, C.c_int32, P, C.c_int32,
                              C.POINTER(C.c_int32)], C.c_int),
+        "fd2_debug_math": ([C.c_int, P, P, C.c_int64, C.c_int], C.c_int),
         "fd2_prf_dims": ([P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "fd2_get_prf": ([P, C.c_int, P], C.c_int),
         "fd2_get_hist_counts": ([P, C.c_int, P], C.c_int),
@@ -126,6 +127,20 @@ def lib() -> C.CDLL:
 def _check(rc: int, what: str) -> None:
     if rc != FD2_OK:
         raise Fd2Error(f"{what}: {lib().fd2_strerror(rc).decode()} ({rc})")
+
+
+# fd2_debug_math function numbers (FD2_MATH_* in fd2.h)
+MATH_FN = {"sin": 0, "cos": 1, "log": 2, "asin": 3, "acos": 4, "sin_turns": 5,
+           "cos_turns": 6, "rsqrt": 7, "sqrt_unit": 8, "angle_over_157": 9}
+
+
+def debug_math(fn: str, x, device: int = 0) -> np.ndarray:
+    """The transport's device arithmetic on the GPU: f(x) elementwise."""
+    xs = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(xs)
+    _check(lib().fd2_debug_math(MATH_FN[fn], xs.ctypes.data, y.ctypes.data, xs.size, device),
+           "fd2_debug_math")
+    return y
 
 
 def as_events(ev) -> np.ndarray:

@@ -181,6 +181,19 @@ int fd2_debug_steps(fd2_ctx *ctx, const fd2_event *events, int64_t n,
                     int32_t max_photons, fd2_step *steps, int32_t max_steps,
                     int32_t *n_steps);
 
+/* Debug: the transport's device arithmetic on n arguments, y[i] = f(x[i])
+ * on GPU `device` (synchronous; no context).  Compared bit for bit with the
+ * same functions on the CPU by the tests.  FD2_MATH_SQRT_UNIT is defined for
+ * x = +-0, x >= 2^-96, +inf and NaN; FD2_MATH_ANGLE_OVER_157 gives 1 when
+ * acos x < 0 or > 1.57 (the new-column and reflection angle tests), else 0. */
+enum {
+  FD2_MATH_SIN = 0, FD2_MATH_COS = 1, FD2_MATH_LOG = 2, FD2_MATH_ASIN = 3,
+  FD2_MATH_ACOS = 4, FD2_MATH_SIN_TURNS = 5, FD2_MATH_COS_TURNS = 6,
+  FD2_MATH_RSQRT = 7, FD2_MATH_SQRT_UNIT = 8, FD2_MATH_ANGLE_OVER_157 = 9,
+  FD2_MATH_NFN = 10
+};
+int fd2_debug_math(int fn, const float *x, float *y, int64_t n, int device);
+
 /* Tallies of geometry `geom` (synchronises first).  prf: xdim*ydim u64,
  * [i][j] row-major with i the x pixel; hist: mynumhist u32 detections per
  * history; phs: mynumbins bins with the reference's rule

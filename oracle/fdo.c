@@ -1114,6 +1114,12 @@ int fdo_math_eval(int fn, const float *x, float *y, int64_t n) {
       case 5: { float sn, cs; fd2_sincos_turns(x[i], &sn, &cs); y[i] = sn; break; }
       case 6: { float sn, cs; fd2_sincos_turns(x[i], &sn, &cs); y[i] = cs; break; }
       case 7: y[i] = fd2_rsqrt(x[i]); break;
+      case 8: y[i] = fd2_sqrt_unit(x[i]); break;
+      case 9: { /* the angle tests of K:2473 and K:1884 on fd2_acos */
+        const float a = fd2_acos(x[i]);
+        y[i] = (a < 0.0f || a > 1.57f) ? 1.0f : 0.0f;
+        break;
+      }
       default: return -1;
     }
   }

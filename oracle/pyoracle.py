@@ -249,7 +249,8 @@ def counter_stream(seed: int, event: int, photon: int, n: int) -> np.ndarray:
 
 
 MATH_FN = {"sin": 0, "cos": 1, "log": 2, "asin": 3, "acos": 4,
-           "sin_turns": 5, "cos_turns": 6, "rsqrt": 7}
+           "sin_turns": 5, "cos_turns": 6, "rsqrt": 7, "sqrt_unit": 8,
+           "angle_over_157": 9}
 
 
 def math_eval(fn: str, x) -> np.ndarray:

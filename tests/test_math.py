@@ -53,3 +53,29 @@ def test_turns_and_rsqrt_special_values(oracle):
     x = np.concatenate([np.geomspace(1e-30, 1e30, 20001),
                         np.linspace(0.999, 1.001, 20001)]).astype(np.float32)
     assert ulp_err(oracle.math_eval("rsqrt", x), 1.0 / np.sqrt(x.astype(np.float64))).max() <= 2.0
+
+
+def test_angle_test_facts_exhaustive():
+    """Every float in [-1, 1]: the arc-cosine facts the GPU's branch-free
+    1.57 angle test relies on (oracle/check_angle.c)."""
+    import os
+    import subprocess
+    here = os.path.join(os.path.dirname(__file__), "..", "oracle")
+    subprocess.run(["make", "-s", "-C", here, "check_angle"], check=True)
+    env = dict(os.environ, OMP_NUM_THREADS=str(min(8, os.cpu_count() or 1)))
+    r = subprocess.run([os.path.join(here, "check_angle")], capture_output=True, text=True,
+                       env=env, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 violations" in r.stdout
+
+
+def test_angle_and_sqrt_unit_oracle(oracle):
+    """angle_over_157 on fd2_acos and fd2_sqrt_unit (the CPU side of the
+    device functions) at the band edges and special values."""
+    x = np.array([-1.0, 0.0, 7.90e-4, 7.9632671e-4, 8.03e-4, 1.0, 1.0000001, -1.0000001,
+                  np.nan], np.float32)
+    a = oracle.math_eval("angle_over_157", x)
+    assert list(a[:2]) == [1.0, 1.0] and a[2] == 1.0 and a[4] == 0.0 and a[5] == 0.0
+    assert list(a[6:]) == [0.0, 0.0, 0.0]  # NaN angles pass the test
+    s = np.array([0.0, 2.0 ** -24, 0.25, 1.0, np.inf], np.float32)
+    assert np.array_equal(oracle.math_eval("sqrt_unit", s), np.sqrt(s))
