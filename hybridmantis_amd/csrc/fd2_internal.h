@@ -47,6 +47,10 @@ constexpr int kGapBase = kWallCap;
 constexpr int kBackBase = kWallCap + kGapCap;
 constexpr int kQSlots = kWallCap + kGapCap + kBackCap;
 constexpr int kQWords = 23;
+constexpr int kQBlock = 32;  // queue slots per block (word stride 128 bytes)
+static_assert(kQSlots % kQBlock == 0 && kWallCap % kQBlock == 0 && kGapBase % kQBlock == 0 &&
+                  kBackBase % kQBlock == 0,
+              "queue regions are whole blocks");
 constexpr uint32_t kMaxIdlePasses = 256;  // passes without a step or a wall branch
 constexpr int kTileMax = 120;  // 57.6 KiB: within 64 KiB of LDS per workgroup
 

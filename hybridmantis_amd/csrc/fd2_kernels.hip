@@ -273,83 +273,90 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Phot
 }
 
 // Per-wave photon queues in HBM (wall queue, gap queue, backlog: kQSlots
-// entries of kQWords words; word j of slot s at [j * kQSlots + s], so the
-// lanes of a wave touch consecutive words).  An entry is everything a photon
-// in flight carries: position, direction, normal, column axis, loop count
-// and flags, random stream, event, geometry and photon index; gap-queue
-// entries also the wall branch's loop state (PrptState).
+// entries of kQWords words).  Slots come in blocks of kQBlock = 32; word j
+// of slot s is at [(s / 32) * 32 * kQWords + 32 j + s % 32], so the lanes of
+// a wave write whole 128-byte lines and all words of a slot lie within
+// 3 KB of its first (one address per lane, the rest immediate offsets).  An
+// entry is everything a photon in flight carries: position, direction,
+// normal, column axis, loop count and flags, random stream, event, geometry
+// and photon index; gap-queue entries also the wall branch's loop state
+// (PrptState).
+__device__ __forceinline__ size_t q_base(uint32_t slot) {
+  return (size_t)(slot / kQBlock) * (kQBlock * kQWords) + slot % kQBlock;
+}
+
 __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon& p,
                                       const CounterRng& r, uint32_t ev, uint32_t gi, uint32_t k,
                                       bool parked) {
-  uint32_t* w = wq + slot;
-  w[0 * kQSlots] = __float_as_uint(p.px);
-  w[1 * kQSlots] = __float_as_uint(p.py);
-  w[2 * kQSlots] = __float_as_uint(p.pz);
-  w[3 * kQSlots] = __float_as_uint(p.ux);
-  w[4 * kQSlots] = __float_as_uint(p.uy);
-  w[5 * kQSlots] = __float_as_uint(p.uz);
-  w[6 * kQSlots] = __float_as_uint(p.nx);
-  w[7 * kQSlots] = __float_as_uint(p.ny);
-  w[8 * kQSlots] = __float_as_uint(p.nz);
-  w[9 * kQSlots] = __float_as_uint(p.xc);
-  w[10 * kQSlots] = __float_as_uint(p.yc);
+  uint32_t* w = wq + q_base(slot);
+  w[0 * kQBlock] = __float_as_uint(p.px);
+  w[1 * kQBlock] = __float_as_uint(p.py);
+  w[2 * kQBlock] = __float_as_uint(p.pz);
+  w[3 * kQBlock] = __float_as_uint(p.ux);
+  w[4 * kQBlock] = __float_as_uint(p.uy);
+  w[5 * kQBlock] = __float_as_uint(p.uz);
+  w[6 * kQBlock] = __float_as_uint(p.nx);
+  w[7 * kQBlock] = __float_as_uint(p.ny);
+  w[8 * kQBlock] = __float_as_uint(p.nz);
+  w[9 * kQBlock] = __float_as_uint(p.xc);
+  w[10 * kQBlock] = __float_as_uint(p.yc);
   // loops (21 bits) | first | sensor resume | parked | new-column sample due,
   // with its rejected samples (<= 100) in bits 25-31
-  w[11 * kQSlots] = (uint32_t)p.loops | (p.first ? 1u << 21 : 0u) |
+  w[11 * kQBlock] = (uint32_t)p.loops | (p.first ? 1u << 21 : 0u) |
                     (p.fate == kResume ? 1u << 22 : 0u) | (parked ? 1u << 23 : 0u) |
                     (p.fate == kNewColumn ? (1u << 24) | ((uint32_t)p.pix << 25) : 0u);
-  w[12 * kQSlots] = r.s0;
-  w[13 * kQSlots] = r.s1;
-  w[14 * kQSlots] = r.s2;
-  w[15 * kQSlots] = r.s3;
-  w[16 * kQSlots] = ev;
-  w[17 * kQSlots] = gi;
-  w[18 * kQSlots] = k;
+  w[12 * kQBlock] = r.s0;
+  w[13 * kQBlock] = r.s1;
+  w[14 * kQBlock] = r.s2;
+  w[15 * kQBlock] = r.s3;
+  w[16 * kQBlock] = ev;
+  w[17 * kQBlock] = gi;
+  w[18 * kQBlock] = k;
 }
 
 __device__ __forceinline__ void q_put_state(uint32_t* wq, uint32_t slot, const PrptState& st) {
-  uint32_t* w = wq + slot;
-  w[19 * kQSlots] = st.tries;
-  w[20 * kQSlots] = __float_as_uint(st.pr);
-  w[21 * kQSlots] = __float_as_uint(st.pt);
-  w[22 * kQSlots] = __float_as_uint(st.t2);
+  uint32_t* w = wq + q_base(slot);
+  w[19 * kQBlock] = st.tries;
+  w[20 * kQBlock] = __float_as_uint(st.pr);
+  w[21 * kQBlock] = __float_as_uint(st.pt);
+  w[22 * kQBlock] = __float_as_uint(st.t2);
 }
 
 __device__ __forceinline__ void q_get_state(const uint32_t* wq, uint32_t slot, PrptState& st) {
-  const uint32_t* w = wq + slot;
-  st.tries = w[19 * kQSlots];
-  st.pr = __uint_as_float(w[20 * kQSlots]);
-  st.pt = __uint_as_float(w[21 * kQSlots]);
-  st.t2 = __uint_as_float(w[22 * kQSlots]);
+  const uint32_t* w = wq + q_base(slot);
+  st.tries = w[19 * kQBlock];
+  st.pr = __uint_as_float(w[20 * kQBlock]);
+  st.pt = __uint_as_float(w[21 * kQBlock]);
+  st.t2 = __uint_as_float(w[22 * kQBlock]);
 }
 
 __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon& p, CounterRng& r,
                                       uint32_t& ev, uint32_t& gi, uint32_t& k, bool& parked) {
-  const uint32_t* w = wq + slot;
-  p.px = __uint_as_float(w[0 * kQSlots]);
-  p.py = __uint_as_float(w[1 * kQSlots]);
-  p.pz = __uint_as_float(w[2 * kQSlots]);
-  p.ux = __uint_as_float(w[3 * kQSlots]);
-  p.uy = __uint_as_float(w[4 * kQSlots]);
-  p.uz = __uint_as_float(w[5 * kQSlots]);
-  p.nx = __uint_as_float(w[6 * kQSlots]);
-  p.ny = __uint_as_float(w[7 * kQSlots]);
-  p.nz = __uint_as_float(w[8 * kQSlots]);
-  p.xc = __uint_as_float(w[9 * kQSlots]);
-  p.yc = __uint_as_float(w[10 * kQSlots]);
-  const uint32_t f = w[11 * kQSlots];
+  const uint32_t* w = wq + q_base(slot);
+  p.px = __uint_as_float(w[0 * kQBlock]);
+  p.py = __uint_as_float(w[1 * kQBlock]);
+  p.pz = __uint_as_float(w[2 * kQBlock]);
+  p.ux = __uint_as_float(w[3 * kQBlock]);
+  p.uy = __uint_as_float(w[4 * kQBlock]);
+  p.uz = __uint_as_float(w[5 * kQBlock]);
+  p.nx = __uint_as_float(w[6 * kQBlock]);
+  p.ny = __uint_as_float(w[7 * kQBlock]);
+  p.nz = __uint_as_float(w[8 * kQBlock]);
+  p.xc = __uint_as_float(w[9 * kQBlock]);
+  p.yc = __uint_as_float(w[10 * kQBlock]);
+  const uint32_t f = w[11 * kQBlock];
   p.loops = (int32_t)(f & 0x1fffffu);
   p.first = (f >> 21) & 1u;
   p.fate = ((f >> 22) & 1u) ? kResume : ((f >> 24) & 1u) ? kNewColumn : kNone;
   parked = (f >> 23) & 1u;
   p.pix = ((f >> 24) & 1u) ? (int32_t)(f >> 25) : -1;
-  r.s0 = w[12 * kQSlots];
-  r.s1 = w[13 * kQSlots];
-  r.s2 = w[14 * kQSlots];
-  r.s3 = w[15 * kQSlots];
-  ev = w[16 * kQSlots];
-  gi = w[17 * kQSlots];
-  k = w[18 * kQSlots];
+  r.s0 = w[12 * kQBlock];
+  r.s1 = w[13 * kQBlock];
+  r.s2 = w[14 * kQBlock];
+  r.s3 = w[15 * kQBlock];
+  ev = w[16 * kQBlock];
+  gi = w[17 * kQBlock];
+  k = w[18 * kQBlock];
 }
 
 // Moves the wave's photons in flight (stepping or waiting for the wall
