@@ -47,9 +47,6 @@ struct GeomDev {
   float dspan;         // d_max - d_min
   float eps_lt, eps_gt;    // x < eps, x > eps
   float norm_lo, norm_hi;  // |v| < 1 - eps, |v| > 1 + eps
-  // bulk absorption shortcut: a step of length <= abs_ds_cap with uniform
-  // u < abs_u_cap is certainly shorter than -ln(u)/bulk (1% margin)
-  float abs_ds_cap, abs_u_cap;
   int32_t pixel, xdim, ydim, num_hist;
   int32_t gap_fresnel; // (n1/n2) >= 1.57: gap side recomputes Pr/Pt
   int32_t pad0;
@@ -407,9 +404,12 @@ __device__ __forceinline__ bool isotropic(Rng& rng, Photon& p, const GeomDev& g,
   float da = 999999.0f;
   if (g.bulk > 0.0f) {
     const float u = rng.u();
-    // the logarithm only decides the outcome when u is close to 1 or the
-    // step is long; otherwise da > ds is certain and the outcome the same
-    if (ds >= 0.0f && ds <= g.abs_ds_cap && u < g.abs_u_cap)
+    // -ln u >= 1 - u, so the absorption length da = -ln(u)/bulk exceeds a
+    // step shorter than (1 - u)/bulk less a 2^-20 margin (which covers the
+    // logarithm's error and the roundings, checked for every u the kernel
+    // draws); only the remaining steps need the logarithm, and the outcome
+    // is the same
+    if (ds < ((1.0f - u) * -g.neg_inv_bulk) * 0.99999904632568359375f)
       da = 3.0e38f;
     else
       da = g.neg_inv_bulk * fd2_log(u);

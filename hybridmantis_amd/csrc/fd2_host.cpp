@@ -94,12 +94,6 @@ void derive(const fd2_params& p, GeomDev& g) {
   g.r_nudge = float_le((double)g.R - 1e-5);
   g.asin_n2n1 = float_le(std::asin((double)g.ratio21));
   g.asin_n1n2 = g.ratio12 <= 1.0f ? float_le(std::asin((double)g.ratio12)) : NAN;
-  // longest regular in-column step: a chord of a cylinder of radius R and
-  // height H, doubled, plus 10 um
-  g.abs_ds_cap = (float)(2.0 * std::sqrt(4.0 * (double)g.R * g.R + (double)g.H * g.H) + 10.0);
-  g.abs_u_cap = g.bulk > 0.0f
-                    ? float_le(std::exp(-(double)g.bulk * 1.01 * (double)g.abs_ds_cap))
-                    : 0.0f;
   g.pixel = p.detpixel;
   g.xdim = prf_side(p.detlboundx, p.detuboundx, p.detpixel);
   g.ydim = prf_side(p.detlboundy, p.detuboundy, p.detpixel);

@@ -55,18 +55,19 @@ def test_turns_and_rsqrt_special_values(oracle):
     assert ulp_err(oracle.math_eval("rsqrt", x), 1.0 / np.sqrt(x.astype(np.float64))).max() <= 2.0
 
 
-def test_angle_test_facts_exhaustive():
+def test_device_shortcut_facts_exhaustive():
     """Every float in [-1, 1]: the arc-cosine facts the GPU's branch-free
-    1.57 angle test relies on (oracle/check_angle.c)."""
+    1.57 angle test relies on; every uniform the kernel draws: the bound of
+    its bulk-absorption shortcut (oracle/check_device.c)."""
     import os
     import subprocess
     here = os.path.join(os.path.dirname(__file__), "..", "oracle")
-    subprocess.run(["make", "-s", "-C", here, "check_angle"], check=True)
+    subprocess.run(["make", "-s", "-C", here, "check_device"], check=True)
     env = dict(os.environ, OMP_NUM_THREADS=str(min(8, os.cpu_count() or 1)))
-    r = subprocess.run([os.path.join(here, "check_angle")], capture_output=True, text=True,
+    r = subprocess.run([os.path.join(here, "check_device")], capture_output=True, text=True,
                        env=env, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "0 violations" in r.stdout
+    assert r.stdout.count(" 0 violations") == 2, r.stdout
 
 
 def test_angle_and_sqrt_unit_oracle(oracle):
