@@ -49,7 +49,8 @@ struct GeomDev {
   float norm_lo, norm_hi;  // |v| < 1 - eps, |v| > 1 + eps
   int32_t pixel, xdim, ydim, num_hist;
   int32_t gap_fresnel; // (n1/n2) >= 1.57: gap side recomputes Pr/Pt
-  int32_t pad0;
+  float inv_pixel;     // 1/pixel when pixel is a power of two (x/pixel ==
+                       // x*inv_pixel exactly), else 0
   unsigned long long* prf;   // xdim*ydim
   uint32_t* hist;            // num_hist
 };
@@ -156,7 +157,7 @@ struct Photon {
   bool first;         // no boundary interaction yet (num_rebound == 0)
   int32_t loops;      // loop passes so far (kMaxLoops cap)
   int fate;
-  int32_t pix;
+  int32_t pix;        // detected pixel as row << 16 | column (pix_index)
 };
 
 // A new photon of event ev at its emission point.  The first column is
@@ -177,19 +178,33 @@ __device__ __forceinline__ void start_photon(Photon& p, const fd2_event& ev) {
 }
 
 // Detection: pixel floor((x-lb)/pitch), inside iff lb <= x <= ub
-// (main/kernel_cuda_c_ver1_0.cu:3402-3419).  Records the fate and pixel;
-// the kernels add the PRF count and the per-history count (at hist-1) when
-// the photon ends.
+// (main/kernel_cuda_c_ver1_0.cu:3402-3419).  Records the fate and pixel
+// (row << 16 | column, sides <= FD2_MAX_PRF_SIDE); the kernels add the PRF
+// count and the per-history count (at hist-1) when the photon ends.
 __device__ __forceinline__ void tally_detect(Photon& p, const GeomDev& g) {
   p.fate = kDetectOut;
   p.pix = -1;
-  int i = (int)floorf((p.px - g.lbx) / (float)g.pixel);
-  int j = (int)floorf((p.py - g.lby) / (float)g.pixel);
+  const float ax = p.px - g.lbx, ay = p.py - g.lby;
+  float qx, qy;
+  if (g.inv_pixel != 0.0f) {
+    qx = ax * g.inv_pixel;
+    qy = ay * g.inv_pixel;
+  } else {
+    qx = ax / (float)g.pixel;
+    qy = ay / (float)g.pixel;
+  }
+  int i = (int)floorf(qx);
+  int j = (int)floorf(qy);
   if (p.px <= g.ubx && p.py <= g.uby && p.px >= g.lbx && p.py >= g.lby &&
       i >= 0 && i < g.xdim && j >= 0 && j < g.ydim) {
     p.fate = kDetect;
-    p.pix = i * g.ydim + j;
+    p.pix = (i << 16) | j;
   }
+}
+
+// PRF index (row-major, xdim x ydim) of a detected photon's pixel
+__device__ __forceinline__ int32_t pix_index(int32_t pix, const GeomDev& g) {
+  return (pix >> 16) * g.ydim + (pix & 0xFFFF);
 }
 
 // ------------------------------------------------------- vector helpers

@@ -95,6 +95,7 @@ void derive(const fd2_params& p, GeomDev& g) {
   g.asin_n2n1 = float_le(std::asin((double)g.ratio21));
   g.asin_n1n2 = g.ratio12 <= 1.0f ? float_le(std::asin((double)g.ratio12)) : NAN;
   g.pixel = p.detpixel;
+  g.inv_pixel = (p.detpixel & (p.detpixel - 1)) == 0 ? 1.0f / (float)p.detpixel : 0.0f;
   g.xdim = prf_side(p.detlboundx, p.detuboundx, p.detpixel);
   g.ydim = prf_side(p.detlboundy, p.detuboundy, p.detpixel);
   g.num_hist = p.mynumhist;

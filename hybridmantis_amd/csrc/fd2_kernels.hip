@@ -236,15 +236,14 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Phot
     if (p.fate == kDetect) {
       bool in_tile = false;
       if ((int)gi < a.n_tiles) {
-        const int i = p.pix / gt.ydim;
-        const uint32_t ti = (uint32_t)(i - tile_origin(gt.xdim, side));
-        const uint32_t tj = (uint32_t)(p.pix - i * gt.ydim - tile_origin(gt.ydim, side));
+        const uint32_t ti = (uint32_t)((p.pix >> 16) - tile_origin(gt.xdim, side));
+        const uint32_t tj = (uint32_t)((p.pix & 0xFFFF) - tile_origin(gt.ydim, side));
         if (ti < (uint32_t)side && tj < (uint32_t)side) {
           atomicAdd(&s_tile[gi * tile_area + ti * side + tj], 1u);
           in_tile = true;
         }
       }
-      if (!in_tile) atomicAdd(&gt.prf[p.pix], 1ull);
+      if (!in_tile) atomicAdd(&gt.prf[pix_index(p.pix, gt)], 1ull);
     }
     if (hist >= 1 && hist <= gt.num_hist) {
       if (hist == run_hist && (!MULTI || gi == run_g)) {
@@ -265,7 +264,7 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Phot
   if (REC) {
     fd2_photon rec;
     rec.fate = p.fate;
-    rec.pix = p.pix;
+    rec.pix = p.fate == kDetect ? pix_index(p.pix, a.geoms[MULTI ? gi : 0u]) : -1;
     rec.x = det ? p.px : 0.0f;
     rec.y = det ? p.py : 0.0f;
     a.records[a.photon_off[lane_ev] + lane_k] = rec;
@@ -748,12 +747,12 @@ __global__ void __launch_bounds__(64) k_trace_ranecu(
     }
     cnt[fate_counter(p.fate)]++;
     const bool det = p.fate == kDetect || p.fate == kDetectOut;
-    if (p.fate == kDetect) atomicAdd(&g.prf[p.pix], 1ull);
+    if (p.fate == kDetect) atomicAdd(&g.prf[pix_index(p.pix, g)], 1ull);
     if (det && ev.hist >= 1 && ev.hist <= g.num_hist) atomicAdd(&g.hist[ev.hist - 1], 1u);
     if (records) {
       fd2_photon rec;
       rec.fate = p.fate;
-      rec.pix = p.pix;
+      rec.pix = p.fate == kDetect ? pix_index(p.pix, g) : -1;
       rec.x = det ? p.px : 0.0f;
       rec.y = det ? p.py : 0.0f;
       records[photon_off[i] + k] = rec;
