@@ -243,23 +243,30 @@ __device__ __forceinline__ void refract(Photon& p, float t1, float t2,
 
 // RoughSurface (main/kernel_cuda_c_ver1_0.cu:3261-3300): perturbation with
 // z = 2s-1 and radius sqrt(1-s^2) (s, not z), normalised, then
-// n <- normalise(beta*p + n).
+// n <- normalise(beta*p + n).  rough_offset draws the normalised p.
 template <class Rng>
-__device__ __forceinline__ void rough(Rng& rng, float& nx, float& ny, float& nz,
-                                      float beta) {
+__device__ __forceinline__ void rough_offset(Rng& rng, float& qx, float& qy, float& qz) {
   float s = rng.u();
-  float qz = 2.0f * s - 1.0f;
+  qz = 2.0f * s - 1.0f;
   float rr = fd2_sqrt_unit(1.0f - s * s);
   s = rng.u();
   float sn, cs;
   fd2_sincos_turns(s, &sn, &cs);  // azimuth 2 pi s
-  float qx = rr * cs, qy = rr * sn;
+  qx = rr * cs;
+  qy = rr * sn;
   float inv = fd2_rsqrt(qx * qx + qy * qy + qz * qz);
   qx = qx * inv;
   qy = qy * inv;
   qz = qz * inv;
+}
+
+template <class Rng>
+__device__ __forceinline__ void rough(Rng& rng, float& nx, float& ny, float& nz,
+                                      float beta) {
+  float qx, qy, qz;
+  rough_offset(rng, qx, qy, qz);
   float rx = beta * qx + nx, ry = beta * qy + ny, rz = beta * qz + nz;
-  inv = fd2_rsqrt(rx * rx + ry * ry + rz * rz);
+  float inv = fd2_rsqrt(rx * rx + ry * ry + rz * rz);
   nx = rx * inv;
   ny = ry * inv;
   nz = rz * inv;
@@ -482,10 +489,12 @@ template <class Rng>
 __device__ __forceinline__ bool column_normal_sample(Rng& rng, Photon& p, const GeomDev& g,
                                                      float mx, float my, float mz,
                                                      bool always_rough) {
-  float nx = mx, ny = my, nz = mz;
-  rough(rng, nx, ny, nz, 1.0f);
+  // -d perturbed with beta = 1 and projected on xy: the xy part of p + m is
+  // normalised directly (the same direction as normalising p + m first)
+  float qx, qy, qz;
+  rough_offset(rng, qx, qy, qz);
+  float nx = qx + mx, ny = qy + my, nz = 0.0f;
   const float inv = fd2_rsqrt(nx * nx + ny * ny);
-  nz = 0.0f;
   nx = nx * inv;
   ny = ny * inv;
   if (always_rough || g.beta > 0.0f) rough(rng, nx, ny, nz, g.beta);

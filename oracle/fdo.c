@@ -231,7 +231,12 @@ static void redirect(ctx *c, float t1, float t2, int refract) {
 /* K:3261-3300 RoughSurface: perturbation p from two draws (z = 2s-1 but the
  * radius uses sqrt(1-s^2), a non-uniform direction), normalised; then
  * n <- normalise(beta*p + n). */
-static void rough(ctx *c, float *n, float beta) {
+static void rough_n(ctx *c, float *n, float beta, int normalise);
+static void rough(ctx *c, float *n, float beta) { rough_n(c, n, beta, 1); }
+
+/* normalise = 0 (device mode only): n <- beta p + n without the final
+ * normalisation, for callers that normalise a projection of it */
+static void rough_n(ctx *c, float *n, float beta, int normalise) {
   c->t->work[FDO_W_ROUGH]++;
   float s = uni(c);
   float p2 = 2.0f * s - 1.0f;
@@ -263,7 +268,11 @@ static void rough(ctx *c, float *n, float beta) {
   float r0 = beta * p0 + n[0];
   float r1 = beta * p1 + n[1];
   float r2 = beta * p2 + n[2];
-  if (c->dev) {
+  if (c->dev && !normalise) {
+    n[0] = r0;
+    n[1] = r1;
+    n[2] = r2;
+  } else if (c->dev) {
     const float inv = fd2_rsqrt(r0 * r0 + r1 * r1 + r2 * r2);
     n[0] = r0 * inv;
     n[1] = r1 * inv;
@@ -492,7 +501,9 @@ static int column_normal(ctx *c, const float *md, int *tries, int always_rough) 
     n[0] = md[0];
     n[1] = md[1];
     n[2] = md[2];
-    rough(c, n, 1.0f);
+    /* kernel: the xy part of the unnormalised p + (-d) is normalised below,
+     * which is the same direction as normalising p + (-d) first */
+    rough_n(c, n, 1.0f, !c->dev);
     float den = (float)sqrt((double)(n[0] * n[0] + n[1] * n[1]));
     n[2] = 0.0f;
     if (c->dev) {

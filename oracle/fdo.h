@@ -36,7 +36,12 @@ enum { FDO_RNG_RANECU = 0, FDO_RNG_COUNTER = 1 };
  *                      the C library (bit-exact against the reference);
  *   FDO_MATH_DEVICE    every expression in FP32 with the elementary
  *                      functions of hybridmantis_amd/csrc/fd2_math.h, i.e.
- *                      the GPU kernel's arithmetic restated on the CPU. */
+ *                      the GPU kernel's arithmetic restated on the CPU.
+ *                      Where the kernel computes a quantity another way
+ *                      that is equal in exact arithmetic, so does this
+ *                      mode: azimuths as turns, reciprocal square roots
+ *                      for normalisations, and the new-column normal's
+ *                      xy projection taken before normalising. */
 enum { FDO_MATH_FAITHFUL = 0, FDO_MATH_DEVICE = 1 };
 
 /* photon fates (exactly one per transported photon) */
