@@ -154,7 +154,6 @@ struct Photon {
   float ux, uy, uz;   // direction cosines
   float nx, ny, nz;   // surface normal
   float xc, yc;       // current column axis (the event's first column at start)
-  bool first;         // no boundary interaction yet (num_rebound == 0)
   int32_t loops;      // loop passes so far (kMaxLoops cap)
   int fate;
   int32_t pix;        // detected pixel as row << 16 | column (pix_index)
@@ -170,7 +169,6 @@ __device__ __forceinline__ void start_photon(Photon& p, const fd2_event& ev) {
   p.pz = ev.z;
   p.xc = ev.x - ev.col_dx;
   p.yc = ev.y;
-  p.first = true;
   p.loops = 0;
   p.fate = kNone;
   p.pix = -1;
@@ -367,14 +365,18 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
   if (out_of_slab(p, g)) {
     code = 1;
   } else {
+    // Radii from the column axis.  fd2_sqrt_unit equals sqrtf for arguments
+    // >= 2^-96; a smaller one has a root below 2^-48, and the radii are only
+    // compared with r_nudge (~R) and, inside the loop, with an rn above it,
+    // so every decision is the same.
     float dx = p.px - xc, dy = p.py - yc;
-    float rn = sqrtf(dx * dx + dy * dy);
+    float rn = fd2_sqrt_unit(dx * dx + dy * dy);
     int moved = 0, shrunk = 0;
     float step = 0.5f;
     while (rn > g.r_nudge && moved < 10 && shrunk < 10) {
       const float cx = p.px + step * (-p.ux), cy = p.py + step * (-p.uy);
       const float ex = cx - xc, ey = cy - yc;
-      const float rc = sqrtf(ex * ex + ey * ey);
+      const float rc = fd2_sqrt_unit(ex * ex + ey * ey);
       if (rc > rn) {
         step = step / 2.0f;
         shrunk++;
@@ -919,7 +921,6 @@ __device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
       }
     }
     if (fabsf(p.uz) < g.eps_lt) return kTrapped;
-    p.first = false;
     const int b = ba_entry(rng, p, g);
     if (b == kBaEnded) return kEnded;
     if (b == kBaPrpt) return kNeedPrpt;

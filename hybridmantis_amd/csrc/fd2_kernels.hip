@@ -299,9 +299,9 @@ __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon&
   w[8 * kQBlock] = __float_as_uint(p.nz);
   w[9 * kQBlock] = __float_as_uint(p.xc);
   w[10 * kQBlock] = __float_as_uint(p.yc);
-  // loops (21 bits) | first | sensor resume | parked | new-column sample due,
-  // with its rejected samples (<= 100) in bits 25-31
-  w[11 * kQBlock] = (uint32_t)p.loops | (p.first ? 1u << 21 : 0u) |
+  // loops (21 bits) | sensor resume (bit 22) | parked | new-column sample
+  // due, with its rejected samples (<= 100) in bits 25-31
+  w[11 * kQBlock] = (uint32_t)p.loops |
                     (p.fate == kResume ? 1u << 22 : 0u) | (parked ? 1u << 23 : 0u) |
                     (p.fate == kNewColumn ? (1u << 24) | ((uint32_t)p.pix << 25) : 0u);
   w[12 * kQBlock] = r.s0;
@@ -345,7 +345,6 @@ __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon&
   p.yc = __uint_as_float(w[10 * kQBlock]);
   const uint32_t f = w[11 * kQBlock];
   p.loops = (int32_t)(f & 0x1fffffu);
-  p.first = (f >> 21) & 1u;
   p.fate = ((f >> 22) & 1u) ? kResume : ((f >> 24) & 1u) ? kNewColumn : kNone;
   parked = (f >> 23) & 1u;
   p.pix = ((f >> 24) & 1u) ? (int32_t)(f >> 25) : -1;
