@@ -46,7 +46,7 @@ constexpr int kBackCap = 64 + kWallCap + kGapCap;
 constexpr int kGapBase = kWallCap;
 constexpr int kBackBase = kWallCap + kGapCap;
 constexpr int kQSlots = kWallCap + kGapCap + kBackCap;
-constexpr int kQWords = 23;
+constexpr int kQWords = 20;  // 16 photon words + 4 of wall-branch state
 constexpr int kQBlock = 32;  // queue slots per block (word stride 128 bytes)
 static_assert(kQSlots % kQBlock == 0 && kWallCap % kQBlock == 0 && kGapBase % kQBlock == 0 &&
                   kBackBase % kQBlock == 0,

@@ -277,9 +277,9 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Phot
 // a wave write whole 128-byte lines and all words of a slot lie within
 // 3 KB of its first (one address per lane, the rest immediate offsets).  An
 // entry is everything a photon in flight carries: position, direction,
-// normal, column axis, loop count and flags, random stream, event, geometry
-// and photon index; gap-queue entries also the wall branch's loop state
-// (PrptState).
+// column axis, loop count and flags, random stream, event, geometry and
+// photon index (not the surface normal: every path writes it before reading
+// it); gap-queue entries also the wall branch's loop state (PrptState).
 __device__ __forceinline__ size_t q_base(uint32_t slot) {
   return (size_t)(slot / kQBlock) * (kQBlock * kQWords) + slot % kQBlock;
 }
@@ -294,39 +294,36 @@ __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon&
   w[3 * kQBlock] = __float_as_uint(p.ux);
   w[4 * kQBlock] = __float_as_uint(p.uy);
   w[5 * kQBlock] = __float_as_uint(p.uz);
-  w[6 * kQBlock] = __float_as_uint(p.nx);
-  w[7 * kQBlock] = __float_as_uint(p.ny);
-  w[8 * kQBlock] = __float_as_uint(p.nz);
-  w[9 * kQBlock] = __float_as_uint(p.xc);
-  w[10 * kQBlock] = __float_as_uint(p.yc);
+  w[6 * kQBlock] = __float_as_uint(p.xc);
+  w[7 * kQBlock] = __float_as_uint(p.yc);
   // loops (21 bits) | sensor resume (bit 22) | parked | new-column sample
   // due, with its rejected samples (<= 100) in bits 25-31
-  w[11 * kQBlock] = (uint32_t)p.loops |
+  w[8 * kQBlock] = (uint32_t)p.loops |
                     (p.fate == kResume ? 1u << 22 : 0u) | (parked ? 1u << 23 : 0u) |
                     (p.fate == kNewColumn ? (1u << 24) | ((uint32_t)p.pix << 25) : 0u);
-  w[12 * kQBlock] = r.s0;
-  w[13 * kQBlock] = r.s1;
-  w[14 * kQBlock] = r.s2;
-  w[15 * kQBlock] = r.s3;
-  w[16 * kQBlock] = ev;
-  w[17 * kQBlock] = gi;
-  w[18 * kQBlock] = k;
+  w[9 * kQBlock] = r.s0;
+  w[10 * kQBlock] = r.s1;
+  w[11 * kQBlock] = r.s2;
+  w[12 * kQBlock] = r.s3;
+  w[13 * kQBlock] = ev;
+  w[14 * kQBlock] = gi;
+  w[15 * kQBlock] = k;
 }
 
 __device__ __forceinline__ void q_put_state(uint32_t* wq, uint32_t slot, const PrptState& st) {
   uint32_t* w = wq + q_base(slot);
-  w[19 * kQBlock] = st.tries;
-  w[20 * kQBlock] = __float_as_uint(st.pr);
-  w[21 * kQBlock] = __float_as_uint(st.pt);
-  w[22 * kQBlock] = __float_as_uint(st.t2);
+  w[16 * kQBlock] = st.tries;
+  w[17 * kQBlock] = __float_as_uint(st.pr);
+  w[18 * kQBlock] = __float_as_uint(st.pt);
+  w[19 * kQBlock] = __float_as_uint(st.t2);
 }
 
 __device__ __forceinline__ void q_get_state(const uint32_t* wq, uint32_t slot, PrptState& st) {
   const uint32_t* w = wq + q_base(slot);
-  st.tries = w[19 * kQBlock];
-  st.pr = __uint_as_float(w[20 * kQBlock]);
-  st.pt = __uint_as_float(w[21 * kQBlock]);
-  st.t2 = __uint_as_float(w[22 * kQBlock]);
+  st.tries = w[16 * kQBlock];
+  st.pr = __uint_as_float(w[17 * kQBlock]);
+  st.pt = __uint_as_float(w[18 * kQBlock]);
+  st.t2 = __uint_as_float(w[19 * kQBlock]);
 }
 
 __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon& p, CounterRng& r,
@@ -338,23 +335,21 @@ __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon&
   p.ux = __uint_as_float(w[3 * kQBlock]);
   p.uy = __uint_as_float(w[4 * kQBlock]);
   p.uz = __uint_as_float(w[5 * kQBlock]);
-  p.nx = __uint_as_float(w[6 * kQBlock]);
-  p.ny = __uint_as_float(w[7 * kQBlock]);
-  p.nz = __uint_as_float(w[8 * kQBlock]);
-  p.xc = __uint_as_float(w[9 * kQBlock]);
-  p.yc = __uint_as_float(w[10 * kQBlock]);
-  const uint32_t f = w[11 * kQBlock];
+  p.nx = p.ny = p.nz = 0.0f;  // written before it is read
+  p.xc = __uint_as_float(w[6 * kQBlock]);
+  p.yc = __uint_as_float(w[7 * kQBlock]);
+  const uint32_t f = w[8 * kQBlock];
   p.loops = (int32_t)(f & 0x1fffffu);
   p.fate = ((f >> 22) & 1u) ? kResume : ((f >> 24) & 1u) ? kNewColumn : kNone;
   parked = (f >> 23) & 1u;
   p.pix = ((f >> 24) & 1u) ? (int32_t)(f >> 25) : -1;
-  r.s0 = w[12 * kQBlock];
-  r.s1 = w[13 * kQBlock];
-  r.s2 = w[14 * kQBlock];
-  r.s3 = w[15 * kQBlock];
-  ev = w[16 * kQBlock];
-  gi = w[17 * kQBlock];
-  k = w[18 * kQBlock];
+  r.s0 = w[9 * kQBlock];
+  r.s1 = w[10 * kQBlock];
+  r.s2 = w[11 * kQBlock];
+  r.s3 = w[12 * kQBlock];
+  ev = w[13 * kQBlock];
+  gi = w[14 * kQBlock];
+  k = w[15 * kQBlock];
 }
 
 // Moves the wave's photons in flight (stepping or waiting for the wall
