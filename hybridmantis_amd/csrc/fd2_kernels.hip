@@ -284,6 +284,9 @@ __device__ __forceinline__ size_t q_base(uint32_t slot) {
   return (size_t)(slot / kQBlock) * (kQBlock * kQWords) + slot % kQBlock;
 }
 
+// The geometry index is stored only for several geometries per launch
+// (MULTI; else 0) and the photon index only for per-photon records (REC).
+template <bool MULTI, bool REC>
 __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon& p,
                                       const CounterRng& r, uint32_t ev, uint32_t gi, uint32_t k,
                                       bool parked) {
@@ -306,8 +309,8 @@ __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon&
   w[11 * kQBlock] = r.s2;
   w[12 * kQBlock] = r.s3;
   w[13 * kQBlock] = ev;
-  w[14 * kQBlock] = gi;
-  w[15 * kQBlock] = k;
+  if (MULTI) w[14 * kQBlock] = gi;
+  if (REC) w[15 * kQBlock] = k;
 }
 
 __device__ __forceinline__ void q_put_state(uint32_t* wq, uint32_t slot, const PrptState& st) {
@@ -326,6 +329,7 @@ __device__ __forceinline__ void q_get_state(const uint32_t* wq, uint32_t slot, P
   st.t2 = __uint_as_float(w[19 * kQBlock]);
 }
 
+template <bool MULTI, bool REC>
 __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon& p, CounterRng& r,
                                       uint32_t& ev, uint32_t& gi, uint32_t& k, bool& parked) {
   const uint32_t* w = wq + q_base(slot);
@@ -348,13 +352,14 @@ __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon&
   r.s2 = w[11 * kQBlock];
   r.s3 = w[12 * kQBlock];
   ev = w[13 * kQBlock];
-  gi = w[14 * kQBlock];
-  k = w[15 * kQBlock];
+  gi = MULTI ? w[14 * kQBlock] : 0u;
+  k = REC ? w[15 * kQBlock] : 0u;
 }
 
 // Moves the wave's photons in flight (stepping or waiting for the wall
 // queue) to the top of its backlog; false if the backlog would overflow,
 // which the in-flight bound rules out.
+template <bool MULTI, bool REC>
 __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, bool& alive,
                                                 bool& parked, const Photon& p,
                                                 const CounterRng& rng, uint32_t lane_ev,
@@ -364,8 +369,8 @@ __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, 
   const uint32_t m = (uint32_t)__popcll(am);
   if (n_back + m > (uint32_t)kBackCap) return false;
   if (alive) {
-    q_put(wq, kBackBase + n_back + (uint32_t)__popcll(am & lt_mask), p, rng, lane_ev, gi, lane_k,
-          parked);
+    q_put<MULTI, REC>(wq, kBackBase + n_back + (uint32_t)__popcll(am & lt_mask), p, rng,
+                      lane_ev, gi, lane_k, parked);
     alive = false;
     parked = false;
   }
@@ -441,7 +446,7 @@ k_transport(TransportArgs a) {
       const uint32_t take = min((uint32_t)__popcll(need), n_back);
       const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
       if (!alive && rank < take) {
-        q_get(wq, kBackBase + n_back - 1u - rank, p, rng, lane_ev, gi, lane_k, parked);
+        q_get<MULTI, REC>(wq, kBackBase + n_back - 1u - rank, p, rng, lane_ev, gi, lane_k, parked);
         alive = true;
       }
       n_back -= take;
@@ -521,7 +526,7 @@ k_transport(TransportArgs a) {
         const uint32_t k = min((uint32_t)__popcll(pm), (uint32_t)kWallCap - n_wall);
         const uint32_t rank = (uint32_t)__popcll(pm & lt_mask);
         if (alive && parked && rank < k) {
-          q_put(wq, n_wall + rank, p, rng, lane_ev, gi, lane_k, false);
+          q_put<MULTI, REC>(wq, n_wall + rank, p, rng, lane_ev, gi, lane_k, false);
           alive = false;
           parked = false;
         }
@@ -531,7 +536,8 @@ k_transport(TransportArgs a) {
       const bool lanes_idle = __ballot(alive) == 0ull && pool_done && n_back == 0u;
       if (n_wall >= 64u || (lanes_idle && n_wall != 0u)) {
         progress = true;
-        if (!swap_to_backlog(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k, lt_mask)) {
+        if (!swap_to_backlog<MULTI, REC>(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k,
+                                         lt_mask)) {
           if (lane == 0) atomicOr(a.error_flag, 2u);
           break;
         }
@@ -539,7 +545,7 @@ k_transport(TransportArgs a) {
         __threadfence_block();
         if ((uint32_t)lane < take) {
           bool pk;
-          q_get(wq, n_wall - take + (uint32_t)lane, p, rng, lane_ev, gi, lane_k, pk);
+          q_get<MULTI, REC>(wq, n_wall - take + (uint32_t)lane, p, rng, lane_ev, gi, lane_k, pk);
           alive = true;
         }
         n_wall -= take;
@@ -558,7 +564,7 @@ k_transport(TransportArgs a) {
         if (gm != 0ull) {
           if (alive && r == kPrGap) {
             const uint32_t slot = kGapBase + n_gap + (uint32_t)__popcll(gm & lt_mask);
-            q_put(wq, slot, p, rng, lane_ev, gi, lane_k, false);
+            q_put<MULTI, REC>(wq, slot, p, rng, lane_ev, gi, lane_k, false);
             q_put_state(wq, slot, st);
             alive = false;
           }
@@ -574,7 +580,8 @@ k_transport(TransportArgs a) {
       while (n_gap >= 64u || (n_gap != 0u && n_wall == 0u && __ballot(alive) == 0ull &&
                               pool_done && n_back == 0u)) {
         progress = true;
-        if (!swap_to_backlog(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k, lt_mask)) {
+        if (!swap_to_backlog<MULTI, REC>(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k,
+                                         lt_mask)) {
           failed = true;
           break;
         }
@@ -584,7 +591,7 @@ k_transport(TransportArgs a) {
         if ((uint32_t)lane < take) {
           bool pk;
           const uint32_t slot = kGapBase + n_gap - take + (uint32_t)lane;
-          q_get(wq, slot, p, rng, lane_ev, gi, lane_k, pk);
+          q_get<MULTI, REC>(wq, slot, p, rng, lane_ev, gi, lane_k, pk);
           q_get_state(wq, slot, st);
           alive = true;
         }
@@ -601,7 +608,7 @@ k_transport(TransportArgs a) {
         if (gm != 0ull) {
           if (alive && r == kPrGap) {
             const uint32_t slot = kGapBase + n_gap + (uint32_t)__popcll(gm & lt_mask);
-            q_put(wq, slot, p, rng, lane_ev, gi, lane_k, false);
+            q_put<MULTI, REC>(wq, slot, p, rng, lane_ev, gi, lane_k, false);
             q_put_state(wq, slot, st);
             alive = false;
           }
