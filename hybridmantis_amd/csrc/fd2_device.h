@@ -41,8 +41,10 @@ struct GeomDev {
   float r_nudge;       // gt(R - 1e-5): out-of-column threshold
   float depth20;       // H*0.2 (crosstalk band)
   float neg_inv_bulk;  // -1/bulk
-  float asin_n2n1;     // gt(asin(n2/n1)): TIR inside a column
-  float asin_n1n2;     // gt(asin(n1/n2)) (NaN if n1/n2 > 1): TIR from the gap
+  // total internal reflection, theta1 > asin(n2/n1) inside a column and
+  // theta1 > asin(n1/n2) from the gap, as cos(theta1) < threshold: the
+  // smallest float >= cos(asin(.)) (NaN from the gap when n1/n2 > 1)
+  float cos_tir_in, cos_tir_gap;
   float ratio12, ratio21;  // n1/n2, n2/n1
   float dspan;         // d_max - d_min
   float eps_lt, eps_gt;    // x < eps, x > eps
@@ -227,12 +229,11 @@ __device__ __forceinline__ void reflect(Photon& p, const GeomDev& g) {
   renorm(p.ux, p.uy, p.uz, g);
 }
 
-__device__ __forceinline__ void refract(Photon& p, float t1, float t2,
+// refraction with sin(theta1) = s1, sin(theta2) = s2, cos(theta2) = c2
+__device__ __forceinline__ void refract(Photon& p, float s1, float s2, float c2,
                                         const GeomDev& g) {
   float ca = -p.ux * p.nx + -p.uy * p.ny + -p.uz * p.nz;
-  float s2, c2;
-  fd2_sincos(t2, &s2, &c2);
-  float ratio = s2 / fd2_sin(t1);
+  float ratio = s2 / s1;
   p.ux = -p.nx * c2 - ratio * (p.ux + ca * p.nx);
   p.uy = -p.ny * c2 - ratio * (p.uy + ca * p.ny);
   p.uz = -p.nz * c2 - ratio * (p.uz + ca * p.nz);
@@ -675,21 +676,6 @@ top:
   return false;
 }
 
-// Fresnel probabilities (main/kernel_cuda_c_ver1_0.cu:1811-1871) in FP32:
-// Pr = (tan^2(t1-t2)/tan^2(t1+t2) + sin^2(t1-t2)/sin^2(t1+t2)) / 2,
-// Pt = (sin2t1 sin2t2/(sin^2(t1+t2)cos^2(t1-t2)) + sin2t1 sin2t2/sin^2(t1+t2)) / 2
-__device__ __forceinline__ void fresnel_pq(float t1, float t2, float& pr,
-                                           float& pt) {
-  float sm, cm, sp, cp;
-  fd2_sincos(t1 - t2, &sm, &cm);
-  fd2_sincos(t1 + t2, &sp, &cp);
-  const float tm = sm / cm, tp = sp / cp;
-  const float sp2 = sp * sp;
-  pr = 0.5f * ((tm * tm) / (tp * tp) + (sm * sm) / sp2);
-  const float s12 = fd2_sin(2.0f * t1) * fd2_sin(2.0f * t2);
-  pt = 0.5f * (s12 / (sp2 * (cm * cm)) + s12 / sp2);
-}
-
 // boundary_analysis (main/kernel_cuda_c_ver1_0.cu:1582-1958), split up.
 // ba_entry is the frequent part -- the top reflector and the crosstalk
 // decision (K:1610-1652); it returns kBaAlive, kBaEnded, kBaCross when
@@ -744,15 +730,23 @@ __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
 // The reference runs this as one goto loop whose locals persist across its
 // passes: the resample/retry counters, the stale Pr/Pt of the gap side
 // without Fresnel recomputation, and the stale theta2 that a refraction
-// then uses.  PrptState holds them so that, with stop_at_gap, the loop stops
+// then uses.  The Fresnel branch (K:1811-1871) is evaluated from cos(theta1)
+// (the dot product the reference takes acos of) and Snell's law without the
+// angles themselves: sin(theta1) = sqrt(1 - cos^2), sin(theta2) =
+// (n_from/n_to) sin(theta1), cos(theta2) = sqrt(1 - sin^2), Pr =
+// fd2_fresnel_r (Pr + Pt = 1, so no normalisation); theta2 persists as its
+// sine and cosine.  PrptState holds them so that, with stop_at_gap, the loop stops
 // before each flight through the gap (after the refraction out of a column,
 // and after a reflection off the outside of the next column) and returns
 // kPrGap; a later call with inside = false continues with that flight.  The
 // photon's arithmetic and draws are the same whether or not it stops there.
 struct PrptState {
   uint32_t tries;  // theta, smooth, reperturb, normal counters, 8 bits each
-  float pr, pt, t2;
+  float pr;        // reflection probability
+  float s2, c2;    // sin, cos of theta2
 };
+// the reference starts with Pr = 0 and theta2 = 0
+__device__ __forceinline__ PrptState prpt_initial() { return PrptState{0u, 0.0f, 0.0f, 1.0f}; }
 enum PrptResult : int { kPrAlive = 0, kPrEnded = 1, kPrGap = 2 };
 
 template <class Rng>
@@ -761,7 +755,7 @@ __device__ __forceinline__ int prpt_loop(Rng& rng, Photon& p, const GeomDev& g, 
   const float R = g.R;
   int theta_tries = (int)(st.tries & 0xffu), smooth_tries = (int)((st.tries >> 8) & 0xffu);
   int reperturb = (int)((st.tries >> 16) & 0xffu), normal_tries = (int)(st.tries >> 24);
-  float pr = st.pr, pt = st.pt, t1 = 0.0f, t2 = st.t2;
+  float pr = st.pr, s2 = st.s2, c2 = st.c2, s1 = 0.0f;
   float onx, ony, onz, odx, ody, odz;
   float mx, my, mz;
 
@@ -801,46 +795,42 @@ reperturb_point:
   }
 
 no_perturbation:
-  t1 = mx * p.nx + my * p.ny + mz * p.nz;
-  if (t1 > 1.0f || t1 < 0.0f) {
-    if (!inside) {
-      if (column_normal(rng, p, g, mx, my, mz, normal_tries, false)) return kPrEnded;
-    }
-    if (theta_tries < 100) {
-      theta_tries++;
-      goto prpt;
-    }
-    p.fate = kTheta1;
-    return kPrEnded;
-  }
-  t1 = fd2_acos(t1);
-
-  if (inside) {
-    if (t1 > g.asin_n2n1) {
-      pr = 1.0f;
-      pt = 0.0f;
-    } else {
-      if (t1 < g.eps_lt) t1 = 0.00042f;
-      t2 = fd2_asin(g.ratio12 * fd2_sin(t1));
-      fresnel_pq(t1, t2, pr, pt);
-    }
-  } else if (!g.gap_fresnel) {
-    if (t1 > g.asin_n1n2) {
-      pr = 1.0f;
-      pt = 0.0f;
-    }
-  } else if (t1 < g.eps_lt) {
-    t1 = 0.00042f;
-    t2 = fd2_asin(g.ratio12 * fd2_sin(t1));
-    fresnel_pq(t1, t2, pr, pt);
-  } else {
-    t2 = fd2_asin(g.ratio21 * fd2_sin(t1));
-    fresnel_pq(t1, t2, pr, pt);
-  }
   {
-    const float tot = pr + pt;
-    pr = pr / tot;
-    pt = pt / tot;
+    float c1 = mx * p.nx + my * p.ny + mz * p.nz;  // cos(theta1)
+    if (c1 > 1.0f || c1 < 0.0f) {
+      if (!inside) {
+        if (column_normal(rng, p, g, mx, my, mz, normal_tries, false)) return kPrEnded;
+      }
+      if (theta_tries < 100) {
+        theta_tries++;
+        goto prpt;
+      }
+      p.fate = kTheta1;
+      return kPrEnded;
+    }
+    s1 = fd2_sqrt_unit(1.0f - c1 * c1);
+    // theta1 < eps (c1 == 1) is replaced by 0.00042 rad where the reference
+    // computes Pr from the angles
+    float snell = 0.0f;  // n_from / n_to; 0: Pr is not recomputed
+    if (inside) {
+      if (c1 < g.cos_tir_in)
+        pr = 1.0f;
+      else
+        snell = g.ratio12;
+    } else if (!g.gap_fresnel) {
+      if (c1 < g.cos_tir_gap) pr = 1.0f;
+    } else {
+      snell = c1 >= 1.0f ? g.ratio12 : g.ratio21;
+    }
+    if (snell != 0.0f) {
+      if (c1 >= 1.0f) {
+        c1 = FD2_COS_00042;
+        s1 = FD2_SIN_00042;
+      }
+      s2 = snell * s1;
+      c2 = fd2_sqrt_unit(fmaxf(1.0f - s2 * s2, 0.0f));
+      pr = fd2_fresnel_r(c1, s1, c2, s2);
+    }
   }
 
   if (rng.u() < pr) {
@@ -862,7 +852,7 @@ no_perturbation:
     if (!inside) goto gap_flight;
     return kPrAlive;
   }
-  refract(p, t1, t2, g);
+  refract(p, s1, s2, c2, g);
   if (!inside) return kPrAlive;
   inside = false;
 gap_flight:
@@ -870,8 +860,8 @@ gap_flight:
     st.tries = (uint32_t)theta_tries | ((uint32_t)smooth_tries << 8) |
                ((uint32_t)reperturb << 16) | ((uint32_t)normal_tries << 24);
     st.pr = pr;
-    st.pt = pt;
-    st.t2 = t2;
+    st.s2 = s2;
+    st.c2 = c2;
     return kPrGap;
   }
   if (transmit_gap(rng, p, g)) return kPrEnded;
@@ -881,7 +871,7 @@ gap_flight:
 // The whole branch in one call.  true = ended.
 template <class Rng>
 __device__ __forceinline__ bool ba_prpt(Rng& rng, Photon& p, const GeomDev& g) {
-  PrptState st{0u, 0.0f, 0.0f, 0.0f};
+  PrptState st = prpt_initial();
   return prpt_loop(rng, p, g, st, true, false) == kPrEnded;
 }
 

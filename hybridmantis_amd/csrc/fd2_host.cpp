@@ -92,8 +92,11 @@ void derive(const fd2_params& p, GeomDev& g) {
   g.norm_lo = float_ge(1.0 - eps);
   g.norm_hi = float_le(1.0 + eps);
   g.r_nudge = float_le((double)g.R - 1e-5);
-  g.asin_n2n1 = float_le(std::asin((double)g.ratio21));
-  g.asin_n1n2 = g.ratio12 <= 1.0f ? float_le(std::asin((double)g.ratio12)) : NAN;
+  // cos(asin(r)) = sqrt(1 - r^2); c1 < C  <=>  c1 < float_ge(C) for float c1
+  g.cos_tir_in = float_ge(std::sqrt(1.0 - (double)g.ratio21 * (double)g.ratio21));
+  g.cos_tir_gap = g.ratio12 <= 1.0f
+                      ? float_ge(std::sqrt(1.0 - (double)g.ratio12 * (double)g.ratio12))
+                      : NAN;
   g.pixel = p.detpixel;
   g.inv_pixel = (p.detpixel & (p.detpixel - 1)) == 0 ? 1.0f / (float)p.detpixel : 0.0f;
   g.xdim = prf_side(p.detlboundx, p.detuboundx, p.detpixel);

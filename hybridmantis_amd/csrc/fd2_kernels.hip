@@ -317,16 +317,16 @@ __device__ __forceinline__ void q_put_state(uint32_t* wq, uint32_t slot, const P
   uint32_t* w = wq + q_base(slot);
   w[16 * kQBlock] = st.tries;
   w[17 * kQBlock] = __float_as_uint(st.pr);
-  w[18 * kQBlock] = __float_as_uint(st.pt);
-  w[19 * kQBlock] = __float_as_uint(st.t2);
+  w[18 * kQBlock] = __float_as_uint(st.s2);
+  w[19 * kQBlock] = __float_as_uint(st.c2);
 }
 
 __device__ __forceinline__ void q_get_state(const uint32_t* wq, uint32_t slot, PrptState& st) {
   const uint32_t* w = wq + q_base(slot);
   st.tries = w[16 * kQBlock];
   st.pr = __uint_as_float(w[17 * kQBlock]);
-  st.pt = __uint_as_float(w[18 * kQBlock]);
-  st.t2 = __uint_as_float(w[19 * kQBlock]);
+  st.s2 = __uint_as_float(w[18 * kQBlock]);
+  st.c2 = __uint_as_float(w[19 * kQBlock]);
 }
 
 template <bool MULTI, bool REC>
@@ -549,7 +549,7 @@ k_transport(TransportArgs a) {
           alive = true;
         }
         n_wall -= take;
-        PrptState st{0u, 0.0f, 0.0f, 0.0f};
+        PrptState st = prpt_initial();
         int r = kPrAlive;
         if (alive) r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, true, true);
         if (alive && r == kPrEnded) {
@@ -586,7 +586,7 @@ k_transport(TransportArgs a) {
           break;
         }
         const uint32_t take = min(n_gap, 64u);
-        PrptState st{0u, 0.0f, 0.0f, 0.0f};
+        PrptState st = prpt_initial();
         __threadfence_block();
         if ((uint32_t)lane < take) {
           bool pk;

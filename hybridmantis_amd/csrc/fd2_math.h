@@ -18,6 +18,7 @@
  *               root", 2003) and three Newton steps y (3/2 - (x/2) y^2);
  *   fd2_log     x > 0 normal; x = 2^e m, m in [sqrt(1/2), sqrt(2)),
  *               log m = 2s + s z L(z) with s = (m-1)/(m+1), z = s^2;
+ *   fd2_fresnel_r  unpolarised Fresnel reflectance from sines/cosines;
  *   fd2_sqrt_unit  sqrtf for 0 and x >= 2^-96, with fewer instructions
  *               on the GPU (hardware root plus the correctly rounded fix-up);
  *   fd2_asin / fd2_acos  |x| <= 1 (NaN outside), asin x = x + x z A(z) on
@@ -179,6 +180,24 @@ FD2_HD float fd2_sqrt_unit(float x) {
   return sqrtf(x);
 #endif
 }
+
+/* Unpolarised Fresnel reflectance from the sines and cosines of the angles
+ * of incidence (1) and refraction (2):
+ *   R = (Rs + Rp)/2,  Rs = sin^2(t1-t2)/sin^2(t1+t2),  Rp = tan^2(t1-t2)/tan^2(t1+t2)
+ *     = sin^2(t1-t2) (cos^2(t1-t2) + cos^2(t1+t2)) / (2 sin^2(t1+t2) cos^2(t1-t2)),
+ * with the angle sums from the products (one division).  R + T = 1 exactly
+ * for Snell-related angles, so R is the reflection probability. */
+FD2_HD float fd2_fresnel_r(float c1, float s1, float c2, float s2) {
+  const float sm = s1 * c2 - c1 * s2, sp = s1 * c2 + c1 * s2;
+  const float cm = c1 * c2 + s1 * s2, cp = c1 * c2 - s1 * s2;
+  const float sm2 = sm * sm, cm2 = cm * cm;
+  return (sm2 * (cm2 + cp * cp)) / (2.0f * (sp * sp) * cm2);
+}
+
+/* sin and cos of 0.00042 rad, the angle the reference substitutes for an
+ * incidence angle below eps (float of the exact values) */
+#define FD2_SIN_00042 0x1.b866e4p-12f
+#define FD2_COS_00042 0x1.fffffep-1f
 
 FD2_HD float fd2_asin_core(float x) {
   const float z = x * x;

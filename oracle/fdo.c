@@ -210,20 +210,12 @@ static void redirect(ctx *c, float t1, float t2, int refract) {
     u[2] = 2.0f * ca * n[2] + u[2];
   } else {
     c->t->work[FDO_W_REFRACT]++;
-    if (c->dev) {
-      float s2, c2;
-      fd2_sincos(t2, &s2, &c2);
-      float ratio = s2 / fd2_sin(t1);
-      u[0] = -n[0] * c2 - ratio * (u[0] + ca * n[0]);
-      u[1] = -n[1] * c2 - ratio * (u[1] + ca * n[1]);
-      u[2] = -n[2] * c2 - ratio * (u[2] + ca * n[2]);
-    } else {
-      double ct = cos((double)t2);
-      double ratio = sin((double)t2) / sin((double)t1);
-      u[0] = (float)((double)(-n[0]) * ct - ratio * (double)(u[0] + ca * n[0]));
-      u[1] = (float)((double)(-n[1]) * ct - ratio * (double)(u[1] + ca * n[1]));
-      u[2] = (float)((double)(-n[2]) * ct - ratio * (double)(u[2] + ca * n[2]));
-    }
+    /* faithful mode only (device mode: refract_dev) */
+    double ct = cos((double)t2);
+    double ratio = sin((double)t2) / sin((double)t1);
+    u[0] = (float)((double)(-n[0]) * ct - ratio * (double)(u[0] + ca * n[0]));
+    u[1] = (float)((double)(-n[1]) * ct - ratio * (double)(u[1] + ca * n[1]));
+    u[2] = (float)((double)(-n[2]) * ct - ratio * (double)(u[2] + ca * n[2]));
   }
   renorm(c, u);
 }
@@ -689,15 +681,12 @@ top:  /* K:2646-2807: top reflector reached from the gap */
  * (nIC/nC) sin t1 (t1 ~ 0 uses nC/nIC); below 1.57 only the TIR branch may
  * set them and otherwise they keep their previous values. */
 static void fresnel_pq(const ctx *c, float t1, float t2, float *pr, float *pt) {
-  if (c->dev) { /* the kernel's FP32 form */
-    float sm, cm, sp, cp;
-    fd2_sincos(t1 - t2, &sm, &cm);
-    fd2_sincos(t1 + t2, &sp, &cp);
-    const float tm = sm / cm, tq = sp / cp;
-    const float sp2 = sp * sp;
-    *pr = 0.5f * ((tm * tm) / (tq * tq) + (sm * sm) / sp2);
-    const float s12 = fd2_sin(2.0f * t1) * fd2_sin(2.0f * t2);
-    *pt = 0.5f * (s12 / (sp2 * (cm * cm)) + s12 / sp2);
+  if (c->dev) { /* the kernel's form (fresnel_dev): Pr from sines/cosines */
+    float s1, c1, s2, c2;
+    fd2_sincos(t1, &s1, &c1);
+    fd2_sincos(t2, &s2, &c2);
+    *pr = fd2_fresnel_r(c1, s1, c2, s2);
+    *pt = 1.0f - *pr;
     return;
   }
   double dm = (double)(t1 - t2), dp = (double)(t1 + t2);
@@ -706,6 +695,7 @@ static void fresnel_pq(const ctx *c, float t1, float t2, float *pr, float *pt) {
   *pt = (float)(0.5f * (s12 / (pow(sin(dp), 2) * pow(cos(dm), 2)) + s12 / pow(sin(dp), 2)));
 }
 
+/* faithful mode (device mode: fresnel_dev) */
 static void fresnel(ctx *c, int inside, float *t1, float *t2, float *pr, float *pt) {
   const float n1 = c->g->n_C, n2 = c->g->n_IC;
   if (inside) {
@@ -714,8 +704,7 @@ static void fresnel(ctx *c, int inside, float *t1, float *t2, float *pr, float *
       *pt = 0.0f;
     } else {
       if ((double)*t1 < FDO_EPS) *t1 = (float)0.00042;
-      *t2 = c->dev ? fd2_asin((n1 / n2) * fd2_sin(*t1))
-                   : asinf((float)((double)(n1 / n2) * sin((double)*t1)));
+      *t2 = asinf((float)((double)(n1 / n2) * sin((double)*t1)));
       c->t->work[FDO_W_FRESNEL]++;
       fresnel_pq(c, *t1, *t2, pr, pt);
     }
@@ -727,17 +716,69 @@ static void fresnel(ctx *c, int inside, float *t1, float *t2, float *pr, float *
       }
     } else if ((double)*t1 < FDO_EPS) {
       *t1 = (float)0.00042;
-      *t2 = c->dev ? fd2_asin((n1 / n2) * fd2_sin(*t1))
-                   : asinf((float)((double)(n1 / n2) * sin((double)*t1)));
+      *t2 = asinf((float)((double)(n1 / n2) * sin((double)*t1)));
       c->t->work[FDO_W_FRESNEL]++;
       fresnel_pq(c, *t1, *t2, pr, pt);
     } else {
-      *t2 = c->dev ? fd2_asin((n2 / n1) * fd2_sin(*t1))
-                   : asinf((float)((double)(n2 / n1) * sin((double)*t1)));
+      *t2 = asinf((float)((double)(n2 / n1) * sin((double)*t1)));
       c->t->work[FDO_W_FRESNEL]++;
       fresnel_pq(c, *t1, *t2, pr, pt);
     }
   }
+}
+
+/* The kernel's form of fresnel() (device mode): from c1 = cos(theta1) and
+ * Snell's law without the angles -- sin(theta1) = sqrt(1 - c1^2),
+ * sin(theta2) = (n_from/n_to) sin(theta1), cos(theta2) = sqrt(1 - sin^2),
+ * Pr = fd2_fresnel_r, which needs no normalisation (Pr + Pt = 1).  Total
+ * internal reflection as c1 below cos(asin(n2/n1)) (inside) or
+ * cos(asin(n1/n2)) (gap); theta1 < eps (c1 == 1) becomes 0.00042 rad where
+ * Pr is recomputed.  s2, c2 and pr keep their previous values where the
+ * reference keeps theta2 and Pr/Pt. */
+static float cos_asin_ge(double r) {
+  const float f = (float)sqrt(1.0 - r * r);
+  return (double)f < sqrt(1.0 - r * r) ? nextafterf(f, INFINITY) : f;
+}
+
+static void fresnel_dev(ctx *c, int inside, float c1, float *s1, float *s2, float *c2,
+                        float *pr) {
+  const float n1 = c->g->n_C, n2 = c->g->n_IC;
+  *s1 = fd2_sqrt_unit(1.0f - c1 * c1);
+  float snell = 0.0f;
+  if (inside) {
+    if (c1 < cos_asin_ge((double)(n2 / n1)))
+      *pr = 1.0f;
+    else
+      snell = n1 / n2;
+  } else if ((n1 / n2) < 1.57f) {
+    if ((n1 / n2) <= 1.0f && c1 < cos_asin_ge((double)(n1 / n2))) *pr = 1.0f;
+  } else {
+    snell = c1 >= 1.0f ? n1 / n2 : n2 / n1;
+  }
+  if (snell != 0.0f) {
+    if (c1 >= 1.0f) {
+      c1 = FD2_COS_00042;
+      *s1 = FD2_SIN_00042;
+    }
+    *s2 = snell * *s1;
+    *c2 = fd2_sqrt_unit(fmaxf(1.0f - *s2 * *s2, 0.0f));
+    c->t->work[FDO_W_FRESNEL]++;
+    *pr = fd2_fresnel_r(c1, *s1, *c2, *s2);
+  }
+}
+
+/* refraction (trans_dir_cos flag 1) in the kernel's form */
+static void refract_dev(ctx *c, float s1, float s2, float c2) {
+  float *u = c->dcos;
+  const float *n = c->normal;
+  float m[3] = {-u[0], -u[1], -u[2]};
+  float ca = dot3(m, n);
+  c->t->work[FDO_W_REFRACT]++;
+  const float ratio = s2 / s1;
+  u[0] = -n[0] * c2 - ratio * (u[0] + ca * n[0]);
+  u[1] = -n[1] * c2 - ratio * (u[1] + ca * n[1]);
+  u[2] = -n[2] * c2 - ratio * (u[2] + ca * n[2]);
+  renorm(c, u);
 }
 
 /* K:1582-1958 boundary_analysis.  Returns 1 when the photon terminated. */
@@ -787,6 +828,8 @@ static int boundary_analysis(ctx *c) {
   int inside = 1;  /* flag_call_transmit */
   int theta_tries = 0, smooth_tries = 0, reperturb = 0, normal_tries = 0;
   float pr = 0.0f, pt = 0.0f, t1 = 0.0f, t2 = 0.0f;
+  /* device mode: theta1 and theta2 as sines/cosines (theta2 = 0 at first) */
+  float s1 = 0.0f, s2 = 0.0f, c2 = 1.0f;
   float md[3], on[3], od[3];
 
 prpt:  /* K:1679 */
@@ -842,12 +885,15 @@ no_perturbation:  /* K:1746 */
     count(c, FDO_C_THETA1, FDO_FATE_THETA1);
     return 1;
   }
-  t1 = c->dev ? fd2_acos(t1) : acosf(t1);
-
-  fresnel(c, inside, &t1, &t2, &pr, &pt);
-  float tot = pr + pt;
-  pr = pr / tot;
-  pt = pt / tot;
+  if (c->dev) {
+    fresnel_dev(c, inside, t1, &s1, &s2, &c2, &pr);
+  } else {
+    t1 = acosf(t1);
+    fresnel(c, inside, &t1, &t2, &pr, &pt);
+    float tot = pr + pt;
+    pr = pr / tot;
+    pt = pt / tot;
+  }
 
   if (uni(c) < pr) {  /* reflection, K:1874-1932 */
     redirect(c, t1, t2, 0);
@@ -874,7 +920,10 @@ no_perturbation:  /* K:1746 */
     return 0;
   }
   /* transmission, K:1935-1949 */
-  redirect(c, t1, t2, 1);
+  if (c->dev)
+    refract_dev(c, s1, s2, c2);
+  else
+    redirect(c, t1, t2, 1);
   if (inside) {
     inside = 0;
     if (transmit_gap(c)) return 1;

@@ -40,8 +40,10 @@ enum { FDO_RNG_RANECU = 0, FDO_RNG_COUNTER = 1 };
  *                      Where the kernel computes a quantity another way
  *                      that is equal in exact arithmetic, so does this
  *                      mode: azimuths as turns, reciprocal square roots
- *                      for normalisations, and the new-column normal's
- *                      xy projection taken before normalising. */
+ *                      for normalisations, the new-column normal's xy
+ *                      projection taken before normalising, and the
+ *                      Fresnel branch from the sines and cosines of the
+ *                      angles (Snell's law) instead of the angles. */
 enum { FDO_MATH_FAITHFUL = 0, FDO_MATH_DEVICE = 1 };
 
 /* photon fates (exactly one per transported photon) */

@@ -9,7 +9,8 @@ from hybridmantis_amd import events as E
 @pytest.mark.parametrize("math", [0, 1])
 def test_fresnel_energy_conservation_and_normal_incidence(oracle, math):
     # Rs + Ts = 1 and Rp + Tp = 1 for Snell-related angles, so the
-    # reference's Pr + Pt = 1 before its normalisation
+    # reference's Pr + Pt = 1 before its normalisation; the kernel's form
+    # (math 1) computes Pr alone, checked against the closed form in double
     rng = np.random.default_rng(1)
     for n12 in (1.8, 1 / 1.8):
         tmax = np.arcsin(min(1.0, 1.0 / n12)) * 0.999
@@ -17,6 +18,10 @@ def test_fresnel_energy_conservation_and_normal_incidence(oracle, math):
             t2 = np.arcsin(n12 * np.sin(t1))
             pr, pt = oracle.fresnel_probs(np.float32(t1), np.float32(t2), math)
             assert abs(pr + pt - 1.0) < 2e-4
+            a, b = float(np.float32(t1)), float(np.float32(t2))
+            exact = 0.5 * (np.sin(a - b) ** 2 / np.sin(a + b) ** 2 +
+                           np.tan(a - b) ** 2 / np.tan(a + b) ** 2)
+            assert abs(pr - exact) < (2e-4 if math == 0 else 2e-6 + 1e-5 * exact)
     # near normal incidence Pr -> ((n1-n2)/(n1+n2))^2
     t1 = np.float32(0.00042)
     t2 = np.float32(np.arcsin(1.8 * np.sin(0.00042)))
