@@ -3,11 +3,17 @@
 The transport's cost is a sum over primitives k (an in-column step, a
 RoughSurface perturbation, a Fresnel evaluation, ...) of c_k * F_k:
 
-  F_k  FP32 operations of primitive k, counted from the kernel source
-       (hybridmantis_amd/csrc/fd2_device.h): +, -, *, /, sqrt and each
+  F_k  FP32 operations of primitive k in the reference's formulation of it
+       (main/kernel_cuda_c_ver1_0.cu, C branches: the angle of incidence by
+       acos, Snell's law by asin, Fresnel from sines/cosines of the angle
+       sums, a logarithm per absorption draw): +, -, *, /, sqrt and each
        elementary function (sin, cos, log, asin, acos) count 1, a fused
        multiply-add counts 2; comparisons, selects and integer work count 0
-       except the uniform's int->float scaling (1 per draw).
+       except the uniform's int->float scaling (1 per draw).  The kernel
+       gets some primitives with fewer operations (Fresnel from the sines
+       and cosines of the angles, the logarithm only where it can decide the
+       step); the count stays the algorithm's, so achieved FLOP/s compares
+       across kernel versions.
   c_k  mean number of times primitive k runs per photon, measured by the
        oracle's work counters on the same workload (they count exactly the
        same primitives in the same control flow).
