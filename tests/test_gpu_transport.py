@@ -93,10 +93,12 @@ def test_production_kernel_matches_oracle_per_photon(fd2lib, oracle, golden, nam
     assert np.array_equal(rec["x"][det], o.photons["x"][det])
 
 
-def test_statistics_match_faithful_oracle(fd2lib, oracle):
-    """500 histories at 25 keV (~5.5e5 photons), same events and streams;
-    GPU in FP32, oracle in the C path's float/double arithmetic."""
-    p, ev = E.workload("mono25", scale=0.25)
+@pytest.mark.parametrize("scale", [0.25, 1.0])
+def test_statistics_match_faithful_oracle(fd2lib, oracle, scale):
+    """500 and 2,000 histories at 25 keV (~5.5e5 and ~2.2e6 photons, the
+    latter the reference's CPU-runnable case in full), same events and
+    streams; GPU in FP32, oracle in the C path's float/double arithmetic."""
+    p, ev = E.workload("mono25", scale=scale)
     nh = p["num_hist"]
     print(f"[statistics] {len(ev)} events {int(ev['N'].sum())} photons", flush=True)
     with fd2lib.Transport(p) as tr:
