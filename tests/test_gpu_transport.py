@@ -123,6 +123,35 @@ def test_statistics_match_faithful_oracle(fd2lib, oracle, scale):
     assert g_cnt["generated"] == o.counter("generated")
 
 
+def test_statistics_match_reference_path_independent_streams(fd2lib, oracle):
+    """2,000 histories at 25 keV (~2.2e6 photons, the reference's
+    CPU-runnable case): the production kernel (counter-based streams, FP32)
+    against the reference C path itself -- the oracle's faithful arithmetic
+    with the reference's per-event RANECU streams, pinned bit for bit to the
+    compiled reference -- on the same events with independent random
+    streams, so the chi-square p-values are those of independent samples."""
+    p, ev = E.workload("mono25", scale=1.0)
+    nh = p["num_hist"]
+    print(f"[statistics-independent] {len(ev)} events {int(ev['N'].sum())} photons", flush=True)
+    with fd2lib.Transport(p) as tr:
+        tr.submit(ev)
+        g_prf, g_hist, g_phs, g_cnt = tr.prf(), tr.hist_counts(), tr.phs(), tr.counters()
+    o, _ = oracle.run_parallel(p, ev, nh, workers=min(8, os.cpu_count() or 1),
+                               rng=oracle.RNG_RANECU, seed=123456789,
+                               math=oracle.MATH_FAITHFUL)
+    o_phs = oracle.phs(o.hist_count, p["min_ph"], p["max_ph"], p["num_bins"])
+    _, _, p_phs = stats.chi2_two_sample(g_phs.reshape(-1, 40).sum(1),
+                                        o_phs.reshape(-1, 40).sum(1))
+    edges = np.concatenate([np.arange(0, 20, 1.0), np.geomspace(20, 250, 30)])
+    _, _, p_prf = stats.chi2_two_sample(stats.radial_profile(g_prf, (250.5, 250.5), edges),
+                                        stats.radial_profile(o.prf, (250.5, 250.5), edges))
+    print(f"[statistics-independent] p_phs={p_phs:.3g} p_prf={p_prf:.3g}", flush=True)
+    assert p_phs > 0.01 and p_prf > 0.01
+    assert abs(stats.mean_detected(g_hist) / stats.mean_detected(o.hist_count) - 1) < 0.005
+    assert abs(stats.swank(g_hist) / stats.swank(o.hist_count) - 1) < 0.005
+    assert g_cnt["generated"] == o.counter("generated")
+
+
 def test_deterministic_and_split_invariant(fd2lib):
     p, ev = E.workload("mono25", scale=0.2)
     with fd2lib.Transport(p) as tr:
