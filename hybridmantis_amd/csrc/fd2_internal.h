@@ -36,8 +36,10 @@ struct TransportArgs {
 // LDS PRF tiles of the persistent kernel: geometry g < n_tiles counts the
 // detections in a tile_side^2 pixel square centred on its PRF image in LDS
 // (u32) and adds the block's counts to the PRF at the end.
-constexpr int kTileDefault = 96;
-constexpr int kTileBudget = 96 * 96;   // u32 entries per block (36 KiB)
+constexpr int kTileDefault = 88;
+constexpr int kTileBudget = 88 * 88;   // u32 entries per block (30.25 KiB)
+// started photons of a wave's chunk in LDS (direction + stream words)
+constexpr int kStartWords = 7;
 // per-wave photon queues of the persistent kernel: wall queue, gap queue,
 // backlog (slots in that order)
 constexpr int kWallCap = 128;
@@ -52,7 +54,7 @@ static_assert(kQSlots % kQBlock == 0 && kWallCap % kQBlock == 0 && kGapBase % kQ
                   kBackBase % kQBlock == 0,
               "queue regions are whole blocks");
 constexpr uint32_t kMaxIdlePasses = 256;  // passes without a step or a wall branch
-constexpr int kTileMax = 120;  // 57.6 KiB: within 64 KiB of LDS per workgroup
+constexpr int kTileMax = 112;  // 49 KiB + 8 KiB static: within 64 KiB per workgroup
 
 struct QueueArgs {
   const fd2_event* events;

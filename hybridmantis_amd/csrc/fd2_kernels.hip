@@ -396,6 +396,10 @@ template <bool MULTI, bool REC, bool RADIAL, bool QUEUE>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
 k_transport(TransportArgs a) {
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
+  // the wave's current chunk, started ahead with all lanes: photon
+  // chunk_base + j has its first direction in words 0-2 of slot j and its
+  // random stream (after the initial draws) in words 3-6
+  __shared__ uint32_t s_start[kTransportThreads / 64][kStartWords][64];
   extern __shared__ uint32_t s_tile[];  // n_tiles x tile_side^2
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x) s_cnt[i] = 0;
   const int side = a.tile_side;
@@ -414,9 +418,11 @@ k_transport(TransportArgs a) {
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   const uint32_t chunk = 1u << a.chunk_log2;
 
-  // wave-uniform work pool: photons [pool_next, pool_end) of event pool_ev
+  // wave-uniform work pool: photons [pool_next, pool_end) of event pool_ev;
+  // those in [start_base, start_end) (up to 64) are started in s_start
   int64_t pool_ev = 0;
-  int32_t pool_next = 0, pool_end = 0;
+  int32_t pool_next = 0, pool_end = 0, start_base = 0, start_end = 0;
+  uint32_t (*start)[64] = s_start[threadIdx.x >> 6];
   bool pool_done = false;
 
   Photon p;
@@ -475,17 +481,48 @@ k_transport(TransportArgs a) {
           const uint32_t gg = MULTI ? a.events[lo].geom : 0u;
           atomicAdd(&s_cnt[gg * kCounters + FD2_C_GENERATED], (uint32_t)(pool_end - pool_next));
         }
+        start_end = pool_next;
       }
-      const int avail = pool_end - pool_next;
+      if (pool_next >= start_end) {
+        // Seeding a stream (Philox) and drawing the first direction are
+        // long; every lane starts one of the next 64 photons of the chunk
+        // here, at full width, instead of the few lanes that take photons
+        // in a pass.
+        start_base = pool_next;
+        start_end = min(pool_end, pool_next + 64);
+        if (start_base + lane < start_end) {
+          const uint32_t k = (uint32_t)(start_base + lane);
+          CounterRng r;
+          Photon q;
+          r.seed(a.seed, event_base + (uint32_t)pool_ev, k);
+          counter_initial_dir(r, q, k, a.geoms[MULTI ? a.events[pool_ev].geom : 0u]);
+          start[0][lane] = __float_as_uint(q.ux);
+          start[1][lane] = __float_as_uint(q.uy);
+          start[2][lane] = __float_as_uint(q.uz);
+          start[3][lane] = r.s0;
+          start[4][lane] = r.s1;
+          start[5][lane] = r.s2;
+          start[6][lane] = r.s3;
+        }
+        // lanes read other lanes' slots below; the wave's LDS accesses
+        // complete in order
+        __builtin_amdgcn_wave_barrier();
+      }
+      const int avail = start_end - pool_next;  // started, not yet taken
       const int rank = __popcll(need & lt_mask);
       if (!alive && rank < avail) {
         const fd2_event ev = a.events[pool_ev];
         gi = MULTI ? ev.geom : 0u;
-        const GeomDev& g = a.geoms[gi];
         const uint32_t k = (uint32_t)(pool_next + rank);
+        const uint32_t j = k - (uint32_t)start_base;
         start_photon(p, ev);
-        rng.seed(a.seed, event_base + (uint32_t)pool_ev, k);
-        counter_initial_dir(rng, p, k, g);
+        p.ux = __uint_as_float(start[0][j]);
+        p.uy = __uint_as_float(start[1][j]);
+        p.uz = __uint_as_float(start[2][j]);
+        rng.s0 = start[3][j];
+        rng.s1 = start[4][j];
+        rng.s2 = start[5][j];
+        rng.s3 = start[6][j];
         lane_ev = (uint32_t)pool_ev;
         lane_k = k;
         parked = false;

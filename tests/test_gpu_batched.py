@@ -81,7 +81,7 @@ def test_tallies_do_not_depend_on_launch_shape(fd2lib, batch):
     shapes = [dict(), dict(wall_queue=-1),
               dict(chunk=16, waves_per_cu=8, lds_tile=-1, park_lanes=8, refill_lanes=1),
               dict(chunk=256, waves_per_cu=12, lds_tile=32, park_lanes=64, refill_lanes=64),
-              dict(chunk=32, waves_per_cu=16, lds_tile=120, park_lanes=1, refill_lanes=17)]
+              dict(chunk=32, waves_per_cu=16, lds_tile=112, park_lanes=1, refill_lanes=17)]
     ref = None
     for sh in shapes:
         with fd2lib.Transport(plist) as tr:
