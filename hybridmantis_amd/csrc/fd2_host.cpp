@@ -856,7 +856,7 @@ int fd2_autotune(fd2_ctx* c, const fd2_event* d_events, int64_t n, int32_t repea
   static const int chunks[] = {16, 32, 64, 128, 256};
   static const int waves[] = {8, 12, 16};
   static const int parks[] = {8, 16, 32, 48};
-  static const int tiles[] = {-1, 32, 48, 64, kTileDefault};
+  static const int tiles[] = {-1, 32, 48, kTileDefault, 80};
   static const int refills[] = {1, 4, 8, 16, 32};
   static const int queues[] = {-1, 1};
   fd2_tuning cur = c->tune;

@@ -36,8 +36,8 @@ struct TransportArgs {
 // LDS PRF tiles of the persistent kernel: geometry g < n_tiles counts the
 // detections in a tile_side^2 pixel square centred on its PRF image in LDS
 // (u32) and adds the block's counts to the PRF at the end.
-constexpr int kTileDefault = 88;
-constexpr int kTileBudget = 88 * 88;   // u32 entries per block (30.25 KiB)
+constexpr int kTileDefault = 64;
+constexpr int kTileBudget = 64 * 64;   // u32 entries per block (16 KiB)
 // started photons of a wave's chunk in LDS (direction + stream words)
 constexpr int kStartWords = 7;
 // per-wave photon queues of the persistent kernel: wall queue, gap queue,

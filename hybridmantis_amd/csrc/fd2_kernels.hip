@@ -220,19 +220,29 @@ __device__ __forceinline__ int tile_origin(int dim, int side) {
   return dim > side ? (dim - side) >> 1 : 0;
 }
 
+// Adds a lane's run of detections of one event to the event's history
+// count (the event's history and geometry are read here, once per run, not
+// once per detection).
+template <bool MULTI>
+__device__ __forceinline__ void flush_run(const TransportArgs& a, uint32_t run_ev,
+                                          uint32_t run_cnt) {
+  const fd2_event& ev = a.events[run_ev];
+  const int32_t hist = ev.hist;
+  const GeomDev& g = a.geoms[MULTI ? ev.geom : 0u];
+  if (hist >= 1 && hist <= g.num_hist) atomicAdd(&g.hist[hist - 1], run_cnt);
+}
+
 // Tallies of an ended photon: fate counter, PRF pixel (LDS tile or HBM),
-// per-lane run of the history's detections, radial bin, record.
+// per-lane run of the event's detections, radial bin, record.
 template <bool MULTI, bool REC, bool RADIAL>
 __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Photon& p, uint32_t gi,
                                               uint32_t lane_ev, uint32_t lane_k, uint32_t* s_cnt,
                                               uint32_t* s_tile, int side, int tile_area,
-                                              int32_t& run_hist, uint32_t& run_g,
-                                              uint32_t& run_cnt) {
+                                              uint32_t* run) {
   atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
   const bool det = p.fate == kDetect || p.fate == kDetectOut;
   if (det) {
     const GeomDev& gt = a.geoms[MULTI ? gi : 0u];
-    const int32_t hist = a.events[lane_ev].hist;
     if (p.fate == kDetect) {
       bool in_tile = false;
       if ((int)gi < a.n_tiles) {
@@ -245,15 +255,12 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Phot
       }
       if (!in_tile) atomicAdd(&gt.prf[pix_index(p.pix, gt)], 1ull);
     }
-    if (hist >= 1 && hist <= gt.num_hist) {
-      if (hist == run_hist && (!MULTI || gi == run_g)) {
-        ++run_cnt;
-      } else {
-        if (run_cnt) atomicAdd(&a.geoms[MULTI ? run_g : 0u].hist[run_hist - 1], run_cnt);
-        run_hist = hist;
-        run_g = gi;
-        run_cnt = 1;
-      }
+    if (lane_ev == run[0]) {
+      ++run[1];
+    } else {
+      if (run[1]) flush_run<MULTI>(a, run[0], run[1]);
+      run[0] = lane_ev;
+      run[1] = 1;
     }
   }
   if (RADIAL && det) {
@@ -356,6 +363,12 @@ __device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon&
   k = REC ? w[15 * kQBlock] : 0u;
 }
 
+// number of set bits of a wave-wide mask below this lane (v_mbcnt)
+__device__ __forceinline__ uint32_t rank_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 // Moves the wave's photons in flight (stepping or waiting for the wall
 // queue) to the top of its backlog; false if the backlog would overflow,
 // which the in-flight bound rules out.
@@ -363,13 +376,12 @@ template <bool MULTI, bool REC>
 __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, bool& alive,
                                                 bool& parked, const Photon& p,
                                                 const CounterRng& rng, uint32_t lane_ev,
-                                                uint32_t gi, uint32_t lane_k,
-                                                unsigned long long lt_mask) {
+                                                uint32_t gi, uint32_t lane_k) {
   const unsigned long long am = __ballot(alive);
   const uint32_t m = (uint32_t)__popcll(am);
   if (n_back + m > (uint32_t)kBackCap) return false;
   if (alive) {
-    q_put<MULTI, REC>(wq, kBackBase + n_back + (uint32_t)__popcll(am & lt_mask), p, rng,
+    q_put<MULTI, REC>(wq, kBackBase + n_back + rank_below(am), p, rng,
                       lane_ev, gi, lane_k, parked);
     alive = false;
     parked = false;
@@ -407,15 +419,17 @@ k_transport(TransportArgs a) {
   for (int i = threadIdx.x; i < a.n_tiles * tile_area; i += blockDim.x) s_tile[i] = 0;
   __syncthreads();
 
-  // per-lane run of detections of one history, added to the history's
-  // count when the lane moves on to another history (photons of a chunk
-  // share their event)
-  int32_t run_hist = 0;
-  uint32_t run_g = 0, run_cnt = 0;
+  // per-lane run of detections of one event (event, count), added to the
+  // event's history count when the lane's detections move on to another
+  // event (photons of a chunk share their event); in LDS, as it is touched
+  // only when a photon ends
+  __shared__ uint32_t s_run[kTransportThreads][2];
+  uint32_t* run = s_run[threadIdx.x];
+  run[0] = 0u;
+  run[1] = 0u;
   const uint32_t event_base = a.event_base_dev ? *a.event_base_dev : a.event_base;
 
   const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = (1ull << lane) - 1ull;
   const uint32_t chunk = 1u << a.chunk_log2;
 
   // wave-uniform work pool: photons [pool_next, pool_end) of event pool_ev;
@@ -450,7 +464,7 @@ k_transport(TransportArgs a) {
     if (QUEUE && need != 0ull && n_back != 0u) {
       __threadfence_block();  // the wave's queue stores are complete before its loads
       const uint32_t take = min((uint32_t)__popcll(need), n_back);
-      const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+      const uint32_t rank = rank_below(need);
       if (!alive && rank < take) {
         q_get<MULTI, REC>(wq, kBackBase + n_back - 1u - rank, p, rng, lane_ev, gi, lane_k, parked);
         alive = true;
@@ -509,7 +523,7 @@ k_transport(TransportArgs a) {
         __builtin_amdgcn_wave_barrier();
       }
       const int avail = start_end - pool_next;  // started, not yet taken
-      const int rank = __popcll(need & lt_mask);
+      const int rank = (int)rank_below(need);
       if (!alive && rank < avail) {
         const fd2_event ev = a.events[pool_ev];
         gi = MULTI ? ev.geom : 0u;
@@ -553,7 +567,7 @@ k_transport(TransportArgs a) {
     if (QUEUE) {
       if (ended) {
         finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
-                                          tile_area, run_hist, run_g, run_cnt);
+                                          tile_area, run);
         alive = false;
         ended = false;
       }
@@ -561,7 +575,7 @@ k_transport(TransportArgs a) {
       const unsigned long long pm = __ballot(alive && parked);
       if (pm != 0ull) {
         const uint32_t k = min((uint32_t)__popcll(pm), (uint32_t)kWallCap - n_wall);
-        const uint32_t rank = (uint32_t)__popcll(pm & lt_mask);
+        const uint32_t rank = rank_below(pm);
         if (alive && parked && rank < k) {
           q_put<MULTI, REC>(wq, n_wall + rank, p, rng, lane_ev, gi, lane_k, false);
           alive = false;
@@ -573,8 +587,7 @@ k_transport(TransportArgs a) {
       const bool lanes_idle = __ballot(alive) == 0ull && pool_done && n_back == 0u;
       if (n_wall >= 64u || (lanes_idle && n_wall != 0u)) {
         progress = true;
-        if (!swap_to_backlog<MULTI, REC>(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k,
-                                         lt_mask)) {
+        if (!swap_to_backlog<MULTI, REC>(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k)) {
           if (lane == 0) atomicOr(a.error_flag, 2u);
           break;
         }
@@ -591,7 +604,7 @@ k_transport(TransportArgs a) {
         if (alive) r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, true, true);
         if (alive && r == kPrEnded) {
           finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
-                                            tile_area, run_hist, run_g, run_cnt);
+                                            tile_area, run);
           alive = false;
         }
         // refracted out of the column: the gap flight waits in the gap queue
@@ -600,7 +613,7 @@ k_transport(TransportArgs a) {
         const unsigned long long gm = __ballot(alive && r == kPrGap);
         if (gm != 0ull) {
           if (alive && r == kPrGap) {
-            const uint32_t slot = kGapBase + n_gap + (uint32_t)__popcll(gm & lt_mask);
+            const uint32_t slot = kGapBase + n_gap + rank_below(gm);
             q_put<MULTI, REC>(wq, slot, p, rng, lane_ev, gi, lane_k, false);
             q_put_state(wq, slot, st);
             alive = false;
@@ -617,8 +630,7 @@ k_transport(TransportArgs a) {
       while (n_gap >= 64u || (n_gap != 0u && n_wall == 0u && __ballot(alive) == 0ull &&
                               pool_done && n_back == 0u)) {
         progress = true;
-        if (!swap_to_backlog<MULTI, REC>(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k,
-                                         lt_mask)) {
+        if (!swap_to_backlog<MULTI, REC>(wq, n_back, alive, parked, p, rng, lane_ev, gi, lane_k)) {
           failed = true;
           break;
         }
@@ -637,14 +649,14 @@ k_transport(TransportArgs a) {
         if (alive) r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, false, true);
         if (alive && r == kPrEnded) {
           finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
-                                            tile_area, run_hist, run_g, run_cnt);
+                                            tile_area, run);
           alive = false;
         }
         // the slots just read are free again: [n_gap, n_gap + take)
         const unsigned long long gm = __ballot(alive && r == kPrGap);
         if (gm != 0ull) {
           if (alive && r == kPrGap) {
-            const uint32_t slot = kGapBase + n_gap + (uint32_t)__popcll(gm & lt_mask);
+            const uint32_t slot = kGapBase + n_gap + rank_below(gm);
             q_put<MULTI, REC>(wq, slot, p, rng, lane_ev, gi, lane_k, false);
             q_put_state(wq, slot, st);
             alive = false;
@@ -677,13 +689,13 @@ k_transport(TransportArgs a) {
       }
       if (ended) {
         finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
-                                          tile_area, run_hist, run_g, run_cnt);
+                                          tile_area, run);
         alive = false;
       }
     }
   }
 
-  if (run_cnt) atomicAdd(&a.geoms[MULTI ? run_g : 0u].hist[run_hist - 1], run_cnt);
+  if (run[1]) flush_run<MULTI>(a, run[0], run[1]);
   __syncthreads();
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x)
     if (s_cnt[i]) atomicAdd(&a.counters[i], (unsigned long long)s_cnt[i]);

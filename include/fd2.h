@@ -109,8 +109,8 @@ typedef struct fd2_tuning {
   int32_t waves_per_cu;     /* resident waves per CU (workgroups of 4 waves) */
   int32_t lds_tile;         /* side (pixels, <= 112) of the LDS PRF tiles,
                              * centred on each PRF image, of the first
-                             * min(n_geoms, 7744 / side^2) geometries;
-                             * 0 = default (88), -1 = no tiles              */
+                             * min(n_geoms, 4096 / side^2) geometries;
+                             * 0 = default (64), -1 = no tiles              */
   int32_t park_lanes;       /* lanes of a wave that wait for the Fresnel /
                              * inter-column phase before it runs (1-64)     */
   int32_t refill_lanes;     /* free lanes of a wave before new photons are
@@ -310,7 +310,7 @@ void fd2_graph_destroy(fd2_graph *graph);
  * scratch context with the same parameters (the context's tallies are not
  * touched) for candidate shapes -- coordinate descent, two sweeps over chunk
  * {16..256}, waves per CU {8, 12, 16}, parked lanes {8..48}, LDS tile side
- * {none, 32..88}, refill lanes {1..32} and wall queue {off, on}, each the
+ * {none, 32..80}, refill lanes {1..32} and wall queue {off, on}, each the
  * best of `repeats` runs after a warm-up --
  * and sets the fastest on ctx.  Candidates and their times go to results
  * (up to max_results; *n_results = number written). */
