@@ -224,25 +224,28 @@ __device__ __forceinline__ int tile_origin(int dim, int side) {
 // count (the event's history and geometry are read here, once per run, not
 // once per detection).
 template <bool MULTI>
-__device__ __forceinline__ void flush_run(const TransportArgs& a, uint32_t run_ev,
+__device__ __forceinline__ void flush_run(const TransportArgs& a,
+                                          const GeomDev* __restrict__ geoms, uint32_t run_ev,
                                           uint32_t run_cnt) {
   const fd2_event& ev = a.events[run_ev];
   const int32_t hist = ev.hist;
-  const GeomDev& g = a.geoms[MULTI ? ev.geom : 0u];
+  const GeomDev& g = geoms[MULTI ? ev.geom : 0u];
   if (hist >= 1 && hist <= g.num_hist) atomicAdd(&g.hist[hist - 1], run_cnt);
 }
 
 // Tallies of an ended photon: fate counter, PRF pixel (LDS tile or HBM),
 // per-lane run of the event's detections, radial bin, record.
 template <bool MULTI, bool REC, bool RADIAL>
-__device__ __forceinline__ void finish_photon(const TransportArgs& a, const Photon& p, uint32_t gi,
+__device__ __forceinline__ void finish_photon(const TransportArgs& a,
+                                              const GeomDev* __restrict__ geoms, const Photon& p,
+                                              uint32_t gi,
                                               uint32_t lane_ev, uint32_t lane_k, uint32_t* s_cnt,
                                               uint32_t* s_tile, int side, int tile_area,
                                               uint32_t* run) {
   atomicAdd(&s_cnt[gi * kCounters + fate_counter(p.fate)], 1u);
   const bool det = p.fate == kDetect || p.fate == kDetectOut;
   if (det) {
-    const GeomDev& gt = a.geoms[MULTI ? gi : 0u];
+    const GeomDev& gt = geoms[MULTI ? gi : 0u];
     if (p.fate == kDetect) {
       bool in_tile = false;
       if ((int)gi < a.n_tiles) {
@@ -258,7 +261,7 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Phot
     if (lane_ev == run[0]) {
       ++run[1];
     } else {
-      if (run[1]) flush_run<MULTI>(a, run[0], run[1]);
+      if (run[1]) flush_run<MULTI>(a, geoms, run[0], run[1]);
       run[0] = lane_ev;
       run[1] = 1;
     }
@@ -271,7 +274,7 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a, const Phot
   if (REC) {
     fd2_photon rec;
     rec.fate = p.fate;
-    rec.pix = p.fate == kDetect ? pix_index(p.pix, a.geoms[MULTI ? gi : 0u]) : -1;
+    rec.pix = p.fate == kDetect ? pix_index(p.pix, geoms[MULTI ? gi : 0u]) : -1;
     rec.x = det ? p.px : 0.0f;
     rec.y = det ? p.py : 0.0f;
     a.records[a.photon_off[lane_ev] + lane_k] = rec;
@@ -404,9 +407,13 @@ __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, 
 // flight per wave and the backlog (kBackCap) cannot overflow; the gap queue
 // holds at most 63 + 64.  Every photon keeps its own state and random
 // stream, so the results are the same either way.
+// `geoms` is a.geoms again, as a parameter of its own: the kernel never
+// writes the geometry table (only the tallies it points to), and saying so
+// (__restrict__) lets the compiler read its constants with scalar loads
+// instead of reloading them with vector loads around every store.
 template <bool MULTI, bool REC, bool RADIAL, bool QUEUE>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
-k_transport(TransportArgs a) {
+k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
   // the wave's current chunk, started ahead with all lanes: photon
   // chunk_base + j has its first direction in words 0-2 of slot j and its
@@ -509,7 +516,7 @@ k_transport(TransportArgs a) {
           CounterRng r;
           Photon q;
           r.seed(a.seed, event_base + (uint32_t)pool_ev, k);
-          counter_initial_dir(r, q, k, a.geoms[MULTI ? a.events[pool_ev].geom : 0u]);
+          counter_initial_dir(r, q, k, geoms[MULTI ? a.events[pool_ev].geom : 0u]);
           start[0][lane] = __float_as_uint(q.ux);
           start[1][lane] = __float_as_uint(q.uy);
           start[2][lane] = __float_as_uint(q.uz);
@@ -553,7 +560,7 @@ k_transport(TransportArgs a) {
     // the launch rather than spin
     bool progress = __ballot(alive && !parked) != 0ull;
     if (alive && !parked) {
-      const GeomDev& g = a.geoms[MULTI ? gi : 0u];
+      const GeomDev& g = geoms[MULTI ? gi : 0u];
       const int r = hot_step<CounterRng, false>(rng, p, g, &nosink);
       if (r == kEnded) {
         ended = true;
@@ -566,7 +573,7 @@ k_transport(TransportArgs a) {
     }
     if (QUEUE) {
       if (ended) {
-        finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+        finish_photon<MULTI, REC, RADIAL>(a, geoms, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                           tile_area, run);
         alive = false;
         ended = false;
@@ -601,9 +608,9 @@ k_transport(TransportArgs a) {
         n_wall -= take;
         PrptState st = prpt_initial();
         int r = kPrAlive;
-        if (alive) r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, true, true);
+        if (alive) r = prpt_loop(rng, p, geoms[MULTI ? gi : 0u], st, true, true);
         if (alive && r == kPrEnded) {
-          finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+          finish_photon<MULTI, REC, RADIAL>(a, geoms, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                             tile_area, run);
           alive = false;
         }
@@ -646,9 +653,9 @@ k_transport(TransportArgs a) {
         }
         n_gap -= take;
         int r = kPrAlive;
-        if (alive) r = prpt_loop(rng, p, a.geoms[MULTI ? gi : 0u], st, false, true);
+        if (alive) r = prpt_loop(rng, p, geoms[MULTI ? gi : 0u], st, false, true);
         if (alive && r == kPrEnded) {
-          finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+          finish_photon<MULTI, REC, RADIAL>(a, geoms, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                             tile_area, run);
           alive = false;
         }
@@ -682,25 +689,25 @@ k_transport(TransportArgs a) {
       if (parked_mask != 0ull &&
           (__popcll(parked_mask) >= a.park_lanes || __ballot(alive && !parked) == 0ull)) {
         if (parked) {
-          const GeomDev& g = a.geoms[MULTI ? gi : 0u];
+          const GeomDev& g = geoms[MULTI ? gi : 0u];
           ended = ba_prpt(rng, p, g);
           parked = false;
         }
       }
       if (ended) {
-        finish_photon<MULTI, REC, RADIAL>(a, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+        finish_photon<MULTI, REC, RADIAL>(a, geoms, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                           tile_area, run);
         alive = false;
       }
     }
   }
 
-  if (run[1]) flush_run<MULTI>(a, run[0], run[1]);
+  if (run[1]) flush_run<MULTI>(a, geoms, run[0], run[1]);
   __syncthreads();
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x)
     if (s_cnt[i]) atomicAdd(&a.counters[i], (unsigned long long)s_cnt[i]);
   for (int t = 0; t < a.n_tiles; ++t) {
-    const GeomDev& gt = a.geoms[t];
+    const GeomDev& gt = geoms[t];
     const int i0 = tile_origin(gt.xdim, side), j0 = tile_origin(gt.ydim, side);
     for (int e = threadIdx.x; e < tile_area; e += blockDim.x) {
       const uint32_t v = s_tile[t * tile_area + e];
@@ -718,10 +725,10 @@ static void launch_one(const TransportArgs& a, dim3 grid, hipStream_t s) {
   const size_t smem = (size_t)a.n_tiles * a.tile_side * a.tile_side * sizeof(uint32_t);
   if (a.wave_queue)
     hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL, true>), grid, dim3(kTransportThreads),
-                       smem, s, a);
+                       smem, s, a, a.geoms);
   else
     hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL, false>), grid, dim3(kTransportThreads),
-                       smem, s, a);
+                       smem, s, a, a.geoms);
 }
 
 hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
