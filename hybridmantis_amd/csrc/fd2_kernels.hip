@@ -224,10 +224,10 @@ __device__ __forceinline__ int tile_origin(int dim, int side) {
 // count (the event's history and geometry are read here, once per run, not
 // once per detection).
 template <bool MULTI>
-__device__ __forceinline__ void flush_run(const TransportArgs& a,
-                                          const GeomDev* __restrict__ geoms, uint32_t run_ev,
+__device__ __forceinline__ void flush_run(const GeomDev* __restrict__ geoms,
+                                          const fd2_event* __restrict__ events, uint32_t run_ev,
                                           uint32_t run_cnt) {
-  const fd2_event& ev = a.events[run_ev];
+  const fd2_event& ev = events[run_ev];
   const int32_t hist = ev.hist;
   const GeomDev& g = geoms[MULTI ? ev.geom : 0u];
   if (hist >= 1 && hist <= g.num_hist) atomicAdd(&g.hist[hist - 1], run_cnt);
@@ -237,8 +237,9 @@ __device__ __forceinline__ void flush_run(const TransportArgs& a,
 // per-lane run of the event's detections, radial bin, record.
 template <bool MULTI, bool REC, bool RADIAL>
 __device__ __forceinline__ void finish_photon(const TransportArgs& a,
-                                              const GeomDev* __restrict__ geoms, const Photon& p,
-                                              uint32_t gi,
+                                              const GeomDev* __restrict__ geoms,
+                                              const fd2_event* __restrict__ events,
+                                              const Photon& p, uint32_t gi,
                                               uint32_t lane_ev, uint32_t lane_k, uint32_t* s_cnt,
                                               uint32_t* s_tile, int side, int tile_area,
                                               uint32_t* run) {
@@ -261,13 +262,13 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a,
     if (lane_ev == run[0]) {
       ++run[1];
     } else {
-      if (run[1]) flush_run<MULTI>(a, geoms, run[0], run[1]);
+      if (run[1]) flush_run<MULTI>(geoms, events, run[0], run[1]);
       run[0] = lane_ev;
       run[1] = 1;
     }
   }
   if (RADIAL && det) {
-    const fd2_event& ev = a.events[lane_ev];
+    const fd2_event& ev = events[lane_ev];
     atomicAdd(&a.radial[(size_t)lane_ev * FD2_RADIAL_BINS + radial_bin(p.px - ev.x, p.py - ev.y)],
               1u);
   }
@@ -407,13 +408,15 @@ __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, 
 // flight per wave and the backlog (kBackCap) cannot overflow; the gap queue
 // holds at most 63 + 64.  Every photon keeps its own state and random
 // stream, so the results are the same either way.
-// `geoms` is a.geoms again, as a parameter of its own: the kernel never
-// writes the geometry table (only the tallies it points to), and saying so
-// (__restrict__) lets the compiler read its constants with scalar loads
+// `geoms`, `events` and `chunk_off` are a.geoms, a.events and a.chunk_off
+// again, as parameters of their own: the kernel never writes these tables
+// (only the tallies the geometries point to), and saying so (__restrict__)
+// lets the compiler read them with scalar loads where they are uniform
 // instead of reloading them with vector loads around every store.
 template <bool MULTI, bool REC, bool RADIAL, bool QUEUE>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
-k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
+k_transport(TransportArgs a, const GeomDev* __restrict__ geoms,
+            const fd2_event* __restrict__ events, const uint32_t* __restrict__ chunk_off) {
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
   // the wave's current chunk, started ahead with all lanes: photon
   // chunk_base + j has its first direction in words 0-2 of slot j and its
@@ -451,7 +454,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
   bool alive = false;
   bool parked = false;  // waiting for the wall-Fresnel branch
   uint32_t gi = 0;
-  uint32_t lane_ev = 0;  // the lane's event (index into a.events)
+  uint32_t lane_ev = 0;  // the lane's event (index into events)
   uint32_t lane_k = 0;   // photon index within the event
   StepSink nosink{nullptr, nullptr, 0, 0, false};
 
@@ -484,7 +487,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(a.cursor, 1u);
         c = (uint32_t)__builtin_amdgcn_readfirstlane((int)c);
-        if (c >= a.chunk_off[a.n_events]) {
+        if (c >= chunk_off[a.n_events]) {
           pool_done = true;
           break;
         }
@@ -492,14 +495,14 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
         int64_t lo = 0, hi = a.n_events;
         while (hi - lo > 1) {
           const int64_t mid = (lo + hi) >> 1;
-          if (a.chunk_off[mid] <= c) lo = mid; else hi = mid;
+          if (chunk_off[mid] <= c) lo = mid; else hi = mid;
         }
         pool_ev = lo;
-        const int32_t n_ph = a.events[lo].N;
-        pool_next = (int32_t)((c - a.chunk_off[lo]) << a.chunk_log2);
+        const int32_t n_ph = events[lo].N;
+        pool_next = (int32_t)((c - chunk_off[lo]) << a.chunk_log2);
         pool_end = min(n_ph, pool_next + (int32_t)chunk);
         if (lane == 0) {
-          const uint32_t gg = MULTI ? a.events[lo].geom : 0u;
+          const uint32_t gg = MULTI ? events[lo].geom : 0u;
           atomicAdd(&s_cnt[gg * kCounters + FD2_C_GENERATED], (uint32_t)(pool_end - pool_next));
         }
         start_end = pool_next;
@@ -516,7 +519,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
           CounterRng r;
           Photon q;
           r.seed(a.seed, event_base + (uint32_t)pool_ev, k);
-          counter_initial_dir(r, q, k, geoms[MULTI ? a.events[pool_ev].geom : 0u]);
+          counter_initial_dir(r, q, k, geoms[MULTI ? events[pool_ev].geom : 0u]);
           start[0][lane] = __float_as_uint(q.ux);
           start[1][lane] = __float_as_uint(q.uy);
           start[2][lane] = __float_as_uint(q.uz);
@@ -532,7 +535,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
       const int avail = start_end - pool_next;  // started, not yet taken
       const int rank = (int)rank_below(need);
       if (!alive && rank < avail) {
-        const fd2_event ev = a.events[pool_ev];
+        const fd2_event ev = events[pool_ev];
         gi = MULTI ? ev.geom : 0u;
         const uint32_t k = (uint32_t)(pool_next + rank);
         const uint32_t j = k - (uint32_t)start_base;
@@ -566,14 +569,14 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
         ended = true;
       } else if (r == kTrapped) {
         atomicAdd(&s_cnt[gi * kCounters + FD2_C_REGEN], 1u);
-        restart_trapped(rng, p, g, a.events[lane_ev]);
+        restart_trapped(rng, p, g, events[lane_ev]);
       } else if (r == kNeedPrpt) {
         parked = true;
       }
     }
     if (QUEUE) {
       if (ended) {
-        finish_photon<MULTI, REC, RADIAL>(a, geoms, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+        finish_photon<MULTI, REC, RADIAL>(a, geoms, events, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                           tile_area, run);
         alive = false;
         ended = false;
@@ -610,7 +613,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
         int r = kPrAlive;
         if (alive) r = prpt_loop(rng, p, geoms[MULTI ? gi : 0u], st, true, true);
         if (alive && r == kPrEnded) {
-          finish_photon<MULTI, REC, RADIAL>(a, geoms, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+          finish_photon<MULTI, REC, RADIAL>(a, geoms, events, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                             tile_area, run);
           alive = false;
         }
@@ -655,7 +658,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
         int r = kPrAlive;
         if (alive) r = prpt_loop(rng, p, geoms[MULTI ? gi : 0u], st, false, true);
         if (alive && r == kPrEnded) {
-          finish_photon<MULTI, REC, RADIAL>(a, geoms, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+          finish_photon<MULTI, REC, RADIAL>(a, geoms, events, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                             tile_area, run);
           alive = false;
         }
@@ -695,14 +698,14 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms) {
         }
       }
       if (ended) {
-        finish_photon<MULTI, REC, RADIAL>(a, geoms, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
+        finish_photon<MULTI, REC, RADIAL>(a, geoms, events, p, gi, lane_ev, lane_k, s_cnt, s_tile, side,
                                           tile_area, run);
         alive = false;
       }
     }
   }
 
-  if (run[1]) flush_run<MULTI>(a, geoms, run[0], run[1]);
+  if (run[1]) flush_run<MULTI>(geoms, events, run[0], run[1]);
   __syncthreads();
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x)
     if (s_cnt[i]) atomicAdd(&a.counters[i], (unsigned long long)s_cnt[i]);
@@ -725,10 +728,10 @@ static void launch_one(const TransportArgs& a, dim3 grid, hipStream_t s) {
   const size_t smem = (size_t)a.n_tiles * a.tile_side * a.tile_side * sizeof(uint32_t);
   if (a.wave_queue)
     hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL, true>), grid, dim3(kTransportThreads),
-                       smem, s, a, a.geoms);
+                       smem, s, a, a.geoms, a.events, a.chunk_off);
   else
     hipLaunchKernelGGL((k_transport<MULTI, REC, RADIAL, false>), grid, dim3(kTransportThreads),
-                       smem, s, a, a.geoms);
+                       smem, s, a, a.geoms, a.events, a.chunk_off);
 }
 
 hipError_t launch_transport(const TransportArgs& a, bool multi, bool record,
