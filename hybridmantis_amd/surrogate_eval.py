@@ -93,6 +93,8 @@ def evaluate(scale: float = 1.0, net=None, seed: int = 271828182, noise_floor: b
             routing={k: (float(v) if isinstance(v, float) else int(v)) for k, v in st.items()},
         )
         if st["ms_surrogate"] > 0:
+            # the forward's FLOP over the whole network part (features, GEMMs,
+            # sampling, copies): a lower bound on the GEMMs' own rate
             out["mlp_forward_tflops"] = (MLP_FWD_FLOP * st["n_surrogate"] /
                                          (st["ms_surrogate"] * 1e-3) / 1e12)
             out["mlp_forward_mfma_frac"] = out["mlp_forward_tflops"] / BF16_PEAK_TFLOPS
