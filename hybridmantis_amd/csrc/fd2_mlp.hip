@@ -726,8 +726,13 @@ struct SampleArgs {
 // spread over the lanes; each detected photon gets a radius by inverse CDF
 // (uniform in area within its bin) and a uniform azimuth, and is tallied
 // like a Monte Carlo detection.  A workgroup loops over many events and
-// keeps the central part of geometry 0's PRF in LDS.
-__global__ void __launch_bounds__(kSampleThreads) k_surr_sample(SampleArgs a) {
+// keeps the central part of geometry 0's PRF in LDS.  The read-only tables
+// come again as __restrict__ parameters, so that the geometry constants are
+// read with scalar loads rather than reloaded around every tally atomic.
+__global__ void __launch_bounds__(kSampleThreads)
+k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
+              const uint32_t* __restrict__ ids, const float* __restrict__ logits,
+              const fd2::GeomDev* __restrict__ geoms) {
   extern __shared__ uint32_t s_tile[];
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * 3];
   const int T = a.tile, T2 = a.tile * a.tile;
@@ -737,9 +742,9 @@ __global__ void __launch_bounds__(kSampleThreads) k_surr_sample(SampleArgs a) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int waves = gridDim.x * (kSampleThreads / 64);
   for (int e = blockIdx.x * (kSampleThreads / 64) + w; e < a.n; e += waves) {
-    const fd2_event ev = a.events[e];
-    const fd2::GeomDev& g = a.geoms[ev.geom];
-    const float* z = a.logits + (size_t)e * kOutPad;
+    const fd2_event ev = events[e];
+    const fd2::GeomDev& g = geoms[ev.geom];
+    const float* z = logits + (size_t)e * kOutPad;
     const float zl = z[1 + lane];
     float mx = zl;
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
@@ -753,7 +758,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_surr_sample(SampleArgs a) {
     cdf = lane == 63 ? 1.0f : cdf / total;
     const float pdet = 1.0f / (1.0f + expf(-z[0]));
     fd2::CounterRng rng;
-    rng.seed(a.seed ^ 0x53555252u, a.ids[e], (uint32_t)lane);
+    rng.seed(a.seed ^ 0x53555252u, ids[e], (uint32_t)lane);
     int cnt = 0;
     for (int j = lane; j < ev.N; j += 64) cnt += rng.u() < pdet ? 1 : 0;
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
@@ -794,7 +799,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_surr_sample(SampleArgs a) {
   }
   __syncthreads();
   if (T > 0) {
-    const fd2::GeomDev& g0 = a.geoms[0];
+    const fd2::GeomDev& g0 = geoms[0];
     for (int i = threadIdx.x; i < T2; i += blockDim.x) {
       const uint32_t v = s_tile[i];
       if (!v) continue;
@@ -985,7 +990,8 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
       int blocks = dev_cu * 4;
       if (blocks * 4 > b) blocks = (b + 3) / 4;
       hipLaunchKernelGGL(fd2mlp::k_surr_sample, dim3(blocks), dim3(fd2mlp::kSampleThreads),
-                         (size_t)tile * tile * 4, c->stream, a);
+                         (size_t)tile * tile * 4, c->stream, a, a.events, a.ids,
+                         (const float*)a.logits, (const fd2::GeomDev*)a.geoms);
       if (hipGetLastError() != hipSuccess) rc = FD2_E_DEVICE;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = FD2_E_DEVICE;
