@@ -304,7 +304,7 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
   if (!c) return FD2_E_NOMEM;
   c->device = device;
   c->n_geoms = n_geoms;
-  c->tune.chunk = 64;
+  c->tune.chunk = 128;
   c->tune.waves_per_cu = 16;
   c->tune.lds_tile = 0;
   c->tune.park_lanes = 32;
