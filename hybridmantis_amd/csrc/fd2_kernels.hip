@@ -408,15 +408,26 @@ __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, 
 // flight per wave and the backlog (kBackCap) cannot overflow; the gap queue
 // holds at most 63 + 64.  Every photon keeps its own state and random
 // stream, so the results are the same either way.
-// `geoms`, `events` and `chunk_off` are a.geoms, a.events and a.chunk_off
-// again, as parameters of their own: the kernel never writes these tables
+// `geoms_in`, `events` and `chunk_off` are a.geoms, a.events and
+// a.chunk_off again, as parameters of their own: the kernel never writes these tables
 // (only the tallies the geometries point to), and saying so (__restrict__)
 // lets the compiler read them with scalar loads where they are uniform
 // instead of reloading them with vector loads around every store.
 template <bool MULTI, bool REC, bool RADIAL, bool QUEUE>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
-k_transport(TransportArgs a, const GeomDev* __restrict__ geoms,
+k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
             const fd2_event* __restrict__ events, const uint32_t* __restrict__ chunk_off) {
+  // With several geometries a lane's geometry is its own, so their constants
+  // cannot be scalar loads: the table goes to LDS instead (160 B each).
+  static_assert(sizeof(GeomDev) % 4 == 0, "GeomDev copied as words");
+  __shared__ GeomDev s_geoms[MULTI ? FD2_MAX_GEOMS : 1];
+  if (MULTI) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(geoms_in);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(s_geoms);
+    const int nw = a.n_geoms * (int)(sizeof(GeomDev) / 4);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
+  }
+  const GeomDev* __restrict__ geoms = MULTI ? s_geoms : geoms_in;
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
   // the wave's current chunk, started ahead with all lanes: photon
   // chunk_base + j has its first direction in words 0-2 of slot j and its
