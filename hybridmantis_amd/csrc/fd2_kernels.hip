@@ -403,16 +403,18 @@ __device__ __forceinline__ bool swap_to_backlog(uint32_t* wq, uint32_t& n_back, 
 // the gap queue and their flights through the gap run, again 64 at a time,
 // one flight per photon and pass, in a phase of their own (a photon
 // reflected off the next column's outside waits for another flight).  Free
-// lanes refill from the backlog before the
-// work queue, so no more than 64 + kWallCap + kGapCap photons are ever in
-// flight per wave and the backlog (kBackCap) cannot overflow; the gap queue
-// holds at most 63 + 64.  Every photon keeps its own state and random
-// stream, so the results are the same either way.
+// lanes refill from the backlog before the work queue, so no more than
+// 64 + kWallCap + kGapCap photons are ever in flight per wave and the
+// backlog (kBackCap) cannot overflow; the gap queue holds at most 127.
+// Every photon keeps its own state and random stream, so the results are
+// the same either way.
+//
 // `geoms_in`, `events` and `chunk_off` are a.geoms, a.events and
-// a.chunk_off again, as parameters of their own: the kernel never writes these tables
-// (only the tallies the geometries point to), and saying so (__restrict__)
-// lets the compiler read them with scalar loads where they are uniform
-// instead of reloading them with vector loads around every store.
+// a.chunk_off again, as parameters of their own: the kernel never writes
+// these tables (only the tallies the geometries point to), and saying so
+// (__restrict__) lets the compiler read them with scalar loads where they
+// are uniform instead of reloading them with vector loads around every
+// store.
 template <bool MULTI, bool REC, bool RADIAL, bool QUEUE>
 __global__ void __launch_bounds__(kTransportThreads) FD2_TRANSPORT_ATTR
 k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
@@ -429,9 +431,9 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
   }
   const GeomDev* __restrict__ geoms = MULTI ? s_geoms : geoms_in;
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * kCounters];
-  // the wave's current chunk, started ahead with all lanes: photon
-  // chunk_base + j has its first direction in words 0-2 of slot j and its
-  // random stream (after the initial draws) in words 3-6
+  // the next photons of the wave's chunk, started ahead with all lanes:
+  // photon start_base + j has its first direction in words 0-2 of slot j
+  // and its random stream (after the initial draws) in words 3-6
   __shared__ uint32_t s_start[kTransportThreads / 64][kStartWords][64];
   extern __shared__ uint32_t s_tile[];  // n_tiles x tile_side^2
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x) s_cnt[i] = 0;
