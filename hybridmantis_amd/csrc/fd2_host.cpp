@@ -276,6 +276,29 @@ int check_error_flag(fd2_ctx* c) {
 
 }  // namespace
 
+namespace fd2 {
+
+int submit_radial_async(fd2_ctx* c, const fd2_event* events, int64_t n, uint32_t* d_radial) {
+  if (!c || n < 0 || (n > 0 && (!events || !d_radial))) return FD2_E_INVAL;
+  if (n == 0) return FD2_OK;
+  int rc = validate_host(c, events, n);
+  if (rc) return rc;
+  FD2_HIP(hipSetDevice(c->device));
+  // the previous submit may still read the staging buffer
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  rc = grow_stage(c, (size_t)n);
+  if (rc) return rc;
+  rc = grow(c, (size_t)n);
+  if (rc) return rc;
+  std::memcpy(c->h_stage, events, (size_t)n * sizeof(fd2_event));
+  FD2_HIP(hipMemcpyAsync(c->d_events, c->h_stage, (size_t)n * sizeof(fd2_event),
+                         hipMemcpyHostToDevice, c->stream));
+  FD2_HIP(launch_fill_u32(d_radial, (size_t)n * FD2_RADIAL_BINS, c->stream));
+  return enqueue(c, c->d_events, n, false, nullptr, d_radial);
+}
+
+}  // namespace fd2
+
 extern "C" {
 
 const char* fd2_strerror(int status) {
@@ -509,17 +532,10 @@ int fd2_submit_radial(fd2_ctx* c, const fd2_event* events, int64_t n,
   int rc = validate_host(c, events, n);
   if (rc) return rc;
   FD2_HIP(hipSetDevice(c->device));
-  FD2_HIP(hipStreamSynchronize(c->stream));
-  rc = grow(c, (size_t)n);
-  if (rc) return rc;
   const size_t nbins = (size_t)n * FD2_RADIAL_BINS;
   uint32_t* d_rad = nullptr;
   FD2_HIP(hipMalloc(&d_rad, nbins * sizeof(uint32_t)));
-  if (hipMemcpyAsync(c->d_events, events, (size_t)n * sizeof(fd2_event),
-                     hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      launch_fill_u32(d_rad, nbins, c->stream) != hipSuccess)
-    rc = FD2_E_DEVICE;
-  if (!rc) rc = enqueue(c, c->d_events, n, false, nullptr, d_rad);
+  rc = submit_radial_async(c, events, n, d_rad);
   if (!rc && hipMemcpyAsync(radial, d_rad, nbins * sizeof(uint32_t), hipMemcpyDeviceToHost,
                             c->stream) != hipSuccess)
     rc = FD2_E_DEVICE;

@@ -134,3 +134,14 @@ struct fd2_ctx {
   int64_t next_event = 0;
   fd2_tuning tune{};
 };
+
+namespace fd2 {
+
+// Stage n events through the context's pinned buffer and enqueue their
+// transport, with per-event radial tallies into d_radial (n x
+// FD2_RADIAL_BINS u32, cleared here), on the context's stream without
+// waiting for it: the events take the next n event numbers, as in
+// fd2_submit_radial, and fd2_sync waits for them and checks the error flag.
+int submit_radial_async(fd2_ctx* c, const fd2_event* events, int64_t n, uint32_t* d_radial);
+
+}  // namespace fd2

@@ -357,7 +357,9 @@ int ensure_batch(fd2_mlp* p, int B) {
 }
 
 // forward pass on p->x0 (B rows) -> z[3] logits
-int forward(fd2_mlp* p, int B, hipStream_t stream) {
+// forward pass of B rows of p->x0; the output layer goes to `out` (B x
+// kOutPad f32) when given, else to p->z[3]
+int forward(fd2_mlp* p, int B, hipStream_t stream, float* out = nullptr) {
   const int ins[4] = {kInPad, kHidden, kHidden, kHidden};
   const int outs[4] = {kHidden, kHidden, kHidden, kOutPad};
   const uint16_t* in = p->x0;
@@ -372,7 +374,7 @@ int forward(fd2_mlp* p, int B, hipStream_t stream) {
     a.ldb = outs[l];
     a.kchunk = ins[l];
     a.bias = p->w + kBOff[l];
-    a.outf = p->z[l];
+    a.outf = l == 3 && out ? out : p->z[l];
     a.outb = l < 3 ? p->a[l] : nullptr;
     a.ldo = outs[l];
     MLP_HIP((gemm<false, false, kStoreF32>(a, 1, stream)));
@@ -701,6 +703,18 @@ __global__ void k_surr_features(const fd2_event* ev, int n, const fd2::GeomDev* 
   x0[i] = c < kFeat ? f2bf(feature(e, geoms[e.geom], c)) : (uint16_t)0;
 }
 
+// Network events gathered from the whole submitted array by index; their
+// random-stream keys are the global event numbers base + index.
+__global__ void k_surr_gather(const fd2_event* __restrict__ all, const uint32_t* __restrict__ idx,
+                              int64_t n, uint32_t base, fd2_event* __restrict__ out,
+                              uint32_t* __restrict__ ids) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = idx[i];
+  out[i] = all[k];
+  ids[i] = base + k;
+}
+
 // edges of the radial bins: 0, 0.5 * 400^((j-1)/62) for j = 1..63, 300 um
 __device__ __forceinline__ float radial_edge(int j) {
   if (j <= 0) return 0.0f;
@@ -719,7 +733,16 @@ struct SampleArgs {
   uint32_t seed;
   int tile;                  // LDS PRF tile side (geometry 0), 0 = none
   int tile_i0, tile_j0;      // pixel origin of the tile
+  uint32_t* spread;          // kSpreadCopies u32 copies of geometry 0's PRF, or null
 };
+
+// Geometry 0's detections outside the LDS tile go to one of several u32
+// copies of its PRF, picked by workgroup number, so that the atomics of
+// different workgroups (and XCDs, which take workgroups in turn) meet on
+// the same address less often; k_spread_flush adds the copies to the u64
+// PRF.  Each copy holds fewer counts than the photons of one launch, which
+// the host keeps below 2^32.
+constexpr int kSpreadCopies = 8;
 
 // One wave per event: lanes are the 64 radial bins for the softmax and its
 // CDF; the binomial detected count is a sum of N Bernoulli(p_det) draws
@@ -787,7 +810,11 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
         if (ev.geom == 0 && ti >= 0 && ti < T && tj >= 0 && tj < T)
           atomicAdd(&s_tile[ti * T + tj], 1u);
         else
-          atomicAdd(&g.prf[(size_t)pi * g.ydim + pj], 1ull);
+          if (ev.geom == 0 && a.spread)
+            atomicAdd(&a.spread[(size_t)(blockIdx.x % kSpreadCopies) * g.xdim * g.ydim +
+                                (size_t)pi * g.ydim + pj], 1u);
+          else
+            atomicAdd(&g.prf[(size_t)pi * g.ydim + pj], 1ull);
       }
     }
     if (lane == 0) {
@@ -816,6 +843,20 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
   }
 }
 
+// Adds the u32 PRF copies to the u64 PRF and clears them.  The PRF takes
+// atomics: the Monte Carlo launch may be tallying into it at the same time.
+__global__ void k_spread_flush(uint32_t* __restrict__ spread, size_t npix,
+                               unsigned long long* __restrict__ prf) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  unsigned long long sum = 0;
+  for (int c = 0; c < kSpreadCopies; ++c) {
+    sum += spread[(size_t)c * npix + i];
+    spread[(size_t)c * npix + i] = 0u;
+  }
+  if (sum) atomicAdd(&prf[i], sum);
+}
+
 uint32_t mix32(uint32_t x) {  // integer hash (murmur3 finaliser)
   x ^= x >> 16;
   x *= 0x85EBCA6Bu;
@@ -832,6 +873,7 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
                                     fd2_surrogate_stats* st) {
   if (!c || !p || n < 0 || (n > 0 && !events)) return FD2_E_INVAL;
   if (c->device != p->device) return FD2_E_INVAL;
+  if (n > (int64_t)UINT32_MAX) return FD2_E_INVAL;  // network events are routed by u32 index
   fd2_surrogate_cfg cfg;
   if (cfg_in) {
     cfg = *cfg_in;
@@ -849,11 +891,6 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
   std::memset(&stats, 0, sizeof stats);
   const auto t0 = std::chrono::steady_clock::now();
   MLP_HIP(hipSetDevice(c->device));
-  for (int64_t i = 0; i < n; ++i) {
-    if (events[i].N < 0 || events[i].N > FD2_MAX_PHOTONS_PER_EVENT) return FD2_E_TOO_MANY;
-    if (events[i].geom >= (uint32_t)c->n_geoms) return FD2_E_INVAL;
-  }
-  // routing
   bool in_range[FD2_MAX_GEOMS];
   for (int g = 0; g < c->n_geoms; ++g) {
     const fd2_params& q = c->params[g];
@@ -870,13 +907,26 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
                   q.detradius >= cfg.r_min && q.detradius <= cfg.r_max &&
                   refl >= cfg.refl_min && refl <= cfg.refl_max;
   }
-  std::vector<fd2_event> mc, audit, surr;
-  std::vector<uint32_t> surr_ids;
+  // One pass validates and routes.  Monte Carlo and audit events (a few
+  // per cent) are copied; network events are kept as indices into `events`
+  // and gathered on the device.
+  const int64_t base = c->next_event;
+  std::vector<fd2_event> mc, audit;
+  std::vector<uint32_t> surr_idx;
+  surr_idx.reserve((size_t)n);
+  // network events per sampling launch: fewer than 2^32 photons (the u32
+  // PRF tallies of a launch cannot wrap) and at most 2^30 events (int
+  // indexing in the kernel); slice_start holds each launch's first index
+  std::vector<int64_t> slice_start;
+  uint64_t slice_photons = 0;
   const uint64_t audit_cut = (uint64_t)((double)cfg.audit_fraction * 4294967296.0);
+  const uint32_t seed_mix = fd2mlp::mix32(cfg.seed);
   for (int64_t i = 0; i < n; ++i) {
     const fd2_event& e = events[i];
+    if (e.N < 0 || e.N > FD2_MAX_PHOTONS_PER_EVENT) return FD2_E_TOO_MANY;
+    if (e.geom >= (uint32_t)c->n_geoms) return FD2_E_INVAL;
     const fd2::GeomDev& g = c->hgeom[e.geom];
-    const uint32_t gid = (uint32_t)(c->next_event + i);
+    const uint32_t gid = (uint32_t)(base + i);
     const bool near = e.z > g.half_h - cfg.surface_band || e.z < -g.half_h + cfg.surface_band;
     const float ox = e.x - 0.5f * g.xdet, oy = e.y - 0.5f * g.ydet;
     const bool off_centre = ox * ox + oy * oy > cfg.center_band * cfg.center_band;
@@ -884,34 +934,152 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
     if (near || off_centre || off_axis || !in_range[e.geom]) {
       mc.push_back(e);
       stats.photons_mc += e.N;
-    } else if ((uint64_t)fd2mlp::mix32(gid ^ fd2mlp::mix32(cfg.seed)) < audit_cut) {
+    } else if ((uint64_t)fd2mlp::mix32(gid ^ seed_mix) < audit_cut) {
       audit.push_back(e);
       stats.photons_mc += e.N;
     } else {
-      surr.push_back(e);
-      surr_ids.push_back(gid);
+      if (slice_start.empty() || slice_photons + (uint64_t)e.N > 0xFFFFFFFFull ||
+          (int64_t)surr_idx.size() - slice_start.back() >= ((int64_t)1 << 30)) {
+        slice_start.push_back((int64_t)surr_idx.size());
+        slice_photons = 0;
+      }
+      slice_photons += (uint64_t)e.N;
+      surr_idx.push_back((uint32_t)i);
       stats.photons_surrogate += e.N;
     }
   }
+  const int64_t n_mc = (int64_t)mc.size(), n_audit = (int64_t)audit.size();
+  const int64_t ns = (int64_t)surr_idx.size();
   stats.n_events = n;
-  stats.n_mc = (int64_t)mc.size();
-  stats.n_audit = (int64_t)audit.size();
-  stats.n_surrogate = (int64_t)surr.size();
-  const int64_t base = c->next_event;
+  stats.n_mc = n_mc;
+  stats.n_audit = n_audit;
+  stats.n_surrogate = ns;
 
-  // Monte Carlo part (asynchronous on the context's stream)
+  fd2_event* d_all = nullptr;
+  fd2_event* d_ev = nullptr;
+  uint32_t* d_idx = nullptr;
+  uint32_t* d_ids = nullptr;
+  uint32_t* d_rad = nullptr;
+  float* d_logits = nullptr;
+  uint32_t* d_spread = nullptr;
+  // waits for both streams before the buffers go
+  auto release = [&](int status) {
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !status) status = FD2_E_DEVICE;
+    if (hipStreamSynchronize(p->stream) != hipSuccess && !status) status = FD2_E_DEVICE;
+    for (void* b : {(void*)d_all, (void*)d_ev, (void*)d_idx, (void*)d_ids, (void*)d_rad,
+                    (void*)d_logits, (void*)d_spread})
+      if (b) (void)hipFree(b);
+    return status;
+  };
+
+  // Monte Carlo part: the fallback events, then the audit events, in one
+  // launch with per-event radial tallies (only the audit slice is read),
+  // on the context's stream.  The events keep the numbers that separate
+  // submits of the two sets would give them.
   int rc = FD2_OK;
-  const auto tm0 = std::chrono::steady_clock::now();
-  if (!mc.empty()) rc = fd2_submit(c, mc.data(), (int64_t)mc.size());
-  if (rc) return rc;
-  // audit slice: Monte Carlo with per-event radial tallies, and the
-  // network's prediction for the same events
-  if (!audit.empty()) {
-    std::vector<uint32_t> rad(audit.size() * FD2_RADIAL_BINS);
-    rc = fd2_submit_radial(c, audit.data(), (int64_t)audit.size(), rad.data());
-    if (rc) return rc;
-    std::vector<float> feat(audit.size() * kFeat), pred(audit.size() * kTarget);
-    for (size_t i = 0; i < audit.size(); ++i) {
+  mc.insert(mc.end(), audit.begin(), audit.end());
+  if (!mc.empty()) {
+    if (hipMalloc(&d_rad, mc.size() * FD2_RADIAL_BINS * sizeof(uint32_t)) != hipSuccess)
+      return release(FD2_E_NOMEM);
+    rc = fd2::submit_radial_async(c, mc.data(), (int64_t)mc.size(), d_rad);
+    if (rc) return release(rc);
+  }
+
+  // Network part, on the network's own stream: its kernels take the compute
+  // units that the Monte Carlo launch frees as it drains.  The forward runs
+  // in batches into one logits array; one sampling launch then covers all
+  // network events, so that each workgroup flushes its LDS PRF tile once.
+  bool timed_net = false;
+  if (ns > 0) {
+    const int chunk = 65536;
+    rc = ensure_batch(p, chunk);
+    if (rc) return release(rc);
+    if (hipMalloc(&d_all, (size_t)n * sizeof(fd2_event)) != hipSuccess ||
+        hipMalloc(&d_ev, (size_t)ns * sizeof(fd2_event)) != hipSuccess ||
+        hipMalloc(&d_idx, (size_t)ns * 4) != hipSuccess ||
+        hipMalloc(&d_ids, (size_t)ns * 4) != hipSuccess ||
+        hipMalloc(&d_logits, (size_t)ns * kOutPad * sizeof(float)) != hipSuccess)
+      return release(FD2_E_NOMEM);
+    const size_t npix0 = (size_t)c->hgeom[0].xdim * c->hgeom[0].ydim;
+    if (hipMalloc(&d_spread, npix0 * fd2mlp::kSpreadCopies * sizeof(uint32_t)) != hipSuccess)
+      return release(FD2_E_NOMEM);
+    int tile = c->tune.lds_tile > 0 ? c->tune.lds_tile : kTileDefault;
+    const fd2::GeomDev& g0 = c->hgeom[0];
+    if (tile > g0.xdim) tile = g0.xdim;
+    if (tile > g0.ydim) tile = g0.ydim;
+    const size_t smem = (size_t)tile * tile * 4;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fd2mlp::k_surr_sample,
+                                                     fd2mlp::kSampleThreads, smem) != hipSuccess ||
+        per_cu < 1)
+      return release(FD2_E_DEVICE);
+    if (per_cu > 8) per_cu = 8;
+    const int dev_cu = c->n_cu > 0 ? c->n_cu : 1;
+    if (hipEventRecord(p->e0, p->stream) != hipSuccess ||
+        hipMemcpyAsync(d_all, events, (size_t)n * sizeof(fd2_event), hipMemcpyHostToDevice,
+                       p->stream) != hipSuccess ||
+        hipMemcpyAsync(d_idx, surr_idx.data(), (size_t)ns * 4, hipMemcpyHostToDevice,
+                       p->stream) != hipSuccess)
+      return release(FD2_E_DEVICE);
+    if (fd2::launch_fill_u32(d_spread, npix0 * fd2mlp::kSpreadCopies, p->stream) != hipSuccess)
+      return release(FD2_E_DEVICE);
+    hipLaunchKernelGGL(fd2mlp::k_surr_gather, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                       p->stream, d_all, d_idx, ns, (uint32_t)base, d_ev, d_ids);
+    for (int64_t s0 = 0; s0 < ns && rc == FD2_OK; s0 += chunk) {
+      const int b = (int)(ns - s0 < chunk ? ns - s0 : chunk);
+      hipLaunchKernelGGL(fd2mlp::k_surr_features, dim3((b * kInPad + 255) / 256), dim3(256), 0,
+                         p->stream, d_ev + s0, b, c->d_geoms, p->x0);
+      rc = forward(p, b, p->stream, d_logits + (size_t)s0 * kOutPad);
+    }
+    for (size_t k = 0; k < slice_start.size() && rc == FD2_OK; ++k) {
+      const int64_t s0 = slice_start[k];
+      const int64_t s1 = k + 1 < slice_start.size() ? slice_start[k + 1] : ns;
+      const int b = (int)(s1 - s0);
+      fd2mlp::SampleArgs a;
+      a.events = d_ev + s0;
+      a.ids = d_ids + s0;
+      a.logits = d_logits + (size_t)s0 * kOutPad;
+      a.geoms = c->d_geoms;
+      a.counters = c->d_counters;
+      a.n = b;
+      a.n_geoms = c->n_geoms;
+      a.seed = cfg.seed;
+      a.tile = tile;
+      a.tile_i0 = g0.xdim / 2 - tile / 2;
+      a.tile_j0 = g0.ydim / 2 - tile / 2;
+      a.spread = d_spread;
+      const int waves_per_block = fd2mlp::kSampleThreads / 64;
+      int64_t blocks = (int64_t)dev_cu * per_cu;
+      if (blocks * waves_per_block > b) blocks = (b + waves_per_block - 1) / waves_per_block;
+      hipLaunchKernelGGL(fd2mlp::k_surr_sample, dim3((unsigned)blocks),
+                         dim3(fd2mlp::kSampleThreads), smem, p->stream, a, a.events, a.ids,
+                         a.logits, (const fd2::GeomDev*)a.geoms);
+      hipLaunchKernelGGL(fd2mlp::k_spread_flush, dim3((unsigned)((npix0 + 255) / 256)), dim3(256),
+                         0, p->stream, d_spread, npix0, c->d_prf[0]);
+      if (hipGetLastError() != hipSuccess) rc = FD2_E_DEVICE;
+    }
+    if (rc) return release(rc);
+    if (hipEventRecord(p->e1, p->stream) != hipSuccess) return release(FD2_E_DEVICE);
+    timed_net = true;
+  }
+
+  // the Monte Carlo part's end; the audit slice of its radial tallies
+  std::vector<uint32_t> rad((size_t)n_audit * FD2_RADIAL_BINS);
+  if (!mc.empty()) {
+    rc = fd2_sync(c);
+    if (!rc && n_audit > 0 &&
+        (hipMemcpyAsync(rad.data(), d_rad + (size_t)n_mc * FD2_RADIAL_BINS,
+                        rad.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        c->stream) != hipSuccess ||
+         hipStreamSynchronize(c->stream) != hipSuccess))
+      rc = FD2_E_DEVICE;
+    if (rc) return release(rc);
+  }
+  // the network's prediction for the audit events (queued behind the
+  // network part on the network's stream)
+  if (n_audit > 0) {
+    std::vector<float> feat((size_t)n_audit * kFeat), pred((size_t)n_audit * kTarget);
+    for (int64_t i = 0; i < n_audit; ++i) {
       const fd2_event& e = audit[i];
       const fd2::GeomDev& g = c->hgeom[e.geom];
       feat[i * kFeat + 0] = (g.half_h - e.z) / g.H;
@@ -920,10 +1088,10 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
       feat[i * kFeat + 3] = g.R * (1.0f / 5.0f);
       feat[i * kFeat + 4] = 1.0f - g.top_absfrac;
     }
-    rc = fd2_mlp_forward(p, feat.data(), (int64_t)audit.size(), pred.data());
-    if (rc) return rc;
+    rc = fd2_mlp_forward(p, feat.data(), n_audit, pred.data());
+    if (rc) return release(rc);
     std::vector<double> pm(FD2_RADIAL_BINS, 0.0), pp(FD2_RADIAL_BINS, 0.0);
-    for (size_t i = 0; i < audit.size(); ++i) {
+    for (int64_t i = 0; i < n_audit; ++i) {
       double det = 0.0;
       for (int b = 0; b < FD2_RADIAL_BINS; ++b) {
         det += rad[i * FD2_RADIAL_BINS + b];
@@ -943,66 +1111,14 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
       l1 += std::fabs((sm > 0 ? pm[b] / sm : 0) - (sp > 0 ? pp[b] / sp : 0));
     stats.audit_radial_l1 = l1;
   }
-  MLP_HIP(hipStreamSynchronize(c->stream));
-  const auto tm1 = std::chrono::steady_clock::now();
-
-  // network part
-  if (!surr.empty()) {
-    const int64_t ns = (int64_t)surr.size();
-    const int chunk = 65536;
-    rc = ensure_batch(p, chunk);
-    if (rc) return rc;
-    fd2_event* d_ev = nullptr;
-    uint32_t* d_ids = nullptr;
-    MLP_HIP(hipMalloc(&d_ev, (size_t)ns * sizeof(fd2_event)));
-    if (hipMalloc(&d_ids, (size_t)ns * 4) != hipSuccess) {
-      (void)hipFree(d_ev);
-      return FD2_E_NOMEM;
-    }
-    int tile = c->tune.lds_tile > 0 ? c->tune.lds_tile : kTileDefault;
-    const fd2::GeomDev& g0 = c->hgeom[0];
-    if (tile > g0.xdim) tile = g0.xdim;
-    if (tile > g0.ydim) tile = g0.ydim;
-    int dev_cu = c->n_cu > 0 ? c->n_cu : 1;
-    if (hipMemcpyAsync(d_ev, surr.data(), (size_t)ns * sizeof(fd2_event), hipMemcpyHostToDevice,
-                       c->stream) != hipSuccess ||
-        hipMemcpyAsync(d_ids, surr_ids.data(), (size_t)ns * 4, hipMemcpyHostToDevice,
-                       c->stream) != hipSuccess)
-      rc = FD2_E_DEVICE;
-    for (int64_t s0 = 0; s0 < ns && rc == FD2_OK; s0 += chunk) {
-      const int b = (int)(ns - s0 < chunk ? ns - s0 : chunk);
-      hipLaunchKernelGGL(fd2mlp::k_surr_features, dim3((b * kInPad + 255) / 256), dim3(256), 0,
-                         c->stream, d_ev + s0, b, c->d_geoms, p->x0);
-      rc = forward(p, b, c->stream);
-      if (rc) break;
-      fd2mlp::SampleArgs a;
-      a.events = d_ev + s0;
-      a.ids = d_ids + s0;
-      a.logits = p->z[3];
-      a.geoms = c->d_geoms;
-      a.counters = c->d_counters;
-      a.n = b;
-      a.n_geoms = c->n_geoms;
-      a.seed = cfg.seed;
-      a.tile = tile;
-      a.tile_i0 = g0.xdim / 2 - tile / 2;
-      a.tile_j0 = g0.ydim / 2 - tile / 2;
-      int blocks = dev_cu * 4;
-      if (blocks * 4 > b) blocks = (b + 3) / 4;
-      hipLaunchKernelGGL(fd2mlp::k_surr_sample, dim3(blocks), dim3(fd2mlp::kSampleThreads),
-                         (size_t)tile * tile * 4, c->stream, a, a.events, a.ids,
-                         (const float*)a.logits, (const fd2::GeomDev*)a.geoms);
-      if (hipGetLastError() != hipSuccess) rc = FD2_E_DEVICE;
-    }
-    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = FD2_E_DEVICE;
-    (void)hipFree(d_ev);
-    (void)hipFree(d_ids);
-    if (rc) return rc;
-  }
+  rc = release(FD2_OK);
+  if (rc) return rc;
   const auto t1 = std::chrono::steady_clock::now();
   c->next_event = base + n > c->next_event ? base + n : c->next_event;
-  stats.ms_mc = std::chrono::duration<double, std::milli>(tm1 - tm0).count();
-  stats.ms_surrogate = std::chrono::duration<double, std::milli>(t1 - tm1).count();
+  float ms = 0.0f;
+  if (!mc.empty() && hipEventElapsedTime(&ms, c->ev_start, c->ev_stop) == hipSuccess)
+    stats.ms_mc = ms;
+  if (timed_net && hipEventElapsedTime(&ms, p->e0, p->e1) == hipSuccess) stats.ms_surrogate = ms;
   stats.ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (st) *st = stats;
   return FD2_OK;

@@ -284,6 +284,11 @@ typedef struct fd2_surrogate_stats {
   double audit_detected_mc;      /* detections of audit events (MC)         */
   double audit_detected_pred;    /* sum over audit events of N * p_det      */
   double audit_radial_l1;        /* L1 of pooled radial distributions       */
+  /* wall time of the call; GPU time of the Monte Carlo launch (fallback and
+   * audit events, with the queue build); GPU time of the network part
+   * (upload, gather, forward, sampling) on the network's own stream, which
+   * runs alongside the Monte Carlo launch and takes the compute units it
+   * frees as it drains */
   double ms_total, ms_mc, ms_surrogate;
 } fd2_surrogate_stats;
 int fd2_surrogate_submit(fd2_ctx *ctx, fd2_mlp *mlp, const fd2_event *events,

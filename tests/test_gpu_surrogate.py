@@ -160,3 +160,77 @@ def test_trained_surrogate_gain(fd2lib):
     assert r["swank_rel_err"] < 0.01
     assert r["prf_l1"] < 1.5 * r["prf_l1_mc_vs_mc"] + 0.01
     assert r["prf_l1_9um"] < 1.5 * r["prf_l1_9um_mc_vs_mc"] + 0.01
+
+
+def _tallies(tr):
+    return tr.prf(), tr.hist_counts(), tr.counters()
+
+
+def _same(a, b):
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+    assert a[2] == b[2]
+
+
+def test_surrogate_monte_carlo_routes_match_plain_submit(fd2lib):
+    """Events that the surrogate path sends to the Monte Carlo -- all of
+    them through the surface band, or all of them as audit events -- keep
+    their order and event numbers, so the tallies equal a plain submit of
+    the same events bit for bit, and the audit's Monte Carlo detections are
+    the counter's."""
+    from hybridmantis_amd import events as E, surrogate as S
+
+    p, ev = E.workload("mono50", scale=0.002)
+    net = S.load()
+    try:
+        with fd2lib.Transport(p) as tr:
+            tr.submit(ev)
+            plain = _tallies(tr)
+        with fd2lib.Transport(p) as tr:
+            st = fd2lib.surrogate_submit(tr, net, ev, surface_band=1e6)
+            mc = _tallies(tr)
+        assert st["n_mc"] == len(ev) and st["n_surrogate"] == 0 and st["n_audit"] == 0
+        _same(mc, plain)
+        with fd2lib.Transport(p) as tr:
+            st = fd2lib.surrogate_submit(tr, net, ev, surface_band=0.0, center_band=1e6,
+                                         audit_fraction=1.0)
+            au = _tallies(tr)
+        assert st["n_audit"] == len(ev) and st["n_mc"] == 0 and st["n_surrogate"] == 0
+        _same(au, plain)
+        assert st["audit_detected_mc"] == plain[2]["detected"]
+        assert st["audit_detected_pred"] > 0
+    finally:
+        net.close()
+
+
+def test_surrogate_path_is_deterministic(fd2lib):
+    """The Monte Carlo and network parts run side by side on two streams and
+    tally into the same integer arrays: two runs with the same seed agree
+    exactly, and a second call on the same context adds the same events
+    under new event numbers."""
+    from hybridmantis_amd import events as E, surrogate as S
+
+    p, ev = E.workload("mono50", scale=0.004)
+    net = S.load()
+    try:
+        runs = []
+        for _ in range(2):
+            with fd2lib.Transport(p) as tr:
+                st = fd2lib.surrogate_submit(tr, net, ev, audit_fraction=0.05)
+                runs.append((_tallies(tr), st))
+        _same(runs[0][0], runs[1][0])
+        for k in ("n_mc", "n_surrogate", "n_audit", "photons_mc", "photons_surrogate",
+                  "audit_detected_mc", "audit_detected_pred", "audit_radial_l1"):
+            assert runs[0][1][k] == runs[1][1][k], k
+        st = runs[0][1]
+        assert st["n_mc"] > 0 and st["n_audit"] > 0 and st["n_surrogate"] > 0.8 * len(ev)
+        assert st["ms_mc"] > 0 and st["ms_surrogate"] > 0 and st["ms_total"] > 0
+        with fd2lib.Transport(p) as tr:
+            fd2lib.surrogate_submit(tr, net, ev, audit_fraction=0.05)
+            one = _tallies(tr)
+            fd2lib.surrogate_submit(tr, net, ev, audit_fraction=0.05)
+            two = _tallies(tr)
+        assert two[2]["generated"] == 2 * one[2]["generated"]
+        assert not np.array_equal(two[0], 2 * one[0])  # new random streams
+    finally:
+        net.close()
