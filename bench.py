@@ -126,10 +126,11 @@ def surrogate_leg() -> dict:
     r = surrogate_eval.evaluate(1.0)
     keep = ("histories", "events", "photons", "mc_histories_per_s", "surrogate_histories_per_s",
             "speedup", "prf_l1", "prf_l1_9um", "detected_fraction_rel_err", "swank_rel_err",
-            "mean_detected_rel_err", "mlp_forward_tflops", "mlp_forward_mfma_frac")
+            "mean_detected_rel_err")
     out = {k: r[k] for k in keep if k in r}
     rt = r["routing"]
-    out["routing"] = {k: rt[k] for k in ("n_events", "n_mc", "n_surrogate", "n_audit")}
+    out["routing"] = {k: rt[k] for k in ("n_events", "n_mc", "n_surrogate", "n_audit",
+                                         "ms_total", "ms_mc", "ms_surrogate")}
     out["workload"] = "mono50: 1e6 histories at 50 keV, H=500 um, R=5 um"
     return out
 

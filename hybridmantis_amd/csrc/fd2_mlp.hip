@@ -679,6 +679,10 @@ int fd2_mlp_train(fd2_mlp* p, int steps, int B, float lr, uint32_t seed, float* 
 namespace fd2mlp {
 
 constexpr int kSampleThreads = 256;
+// side (pixels) of the sampler's LDS tile of geometry 0's PRF, centred on
+// the detector: the sampler runs after the Monte Carlo launch has drained,
+// so its tile is sized for its own occupancy, not the transport's
+constexpr int kSampleTile = 128;
 constexpr int kTileDefault = 96;
 
 // network features of an event (see fd2.h): depth below the top / H,
@@ -1003,7 +1007,7 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
     const size_t npix0 = (size_t)c->hgeom[0].xdim * c->hgeom[0].ydim;
     if (hipMalloc(&d_spread, npix0 * fd2mlp::kSpreadCopies * sizeof(uint32_t)) != hipSuccess)
       return release(FD2_E_NOMEM);
-    int tile = c->tune.lds_tile > 0 ? c->tune.lds_tile : kTileDefault;
+    int tile = fd2mlp::kSampleTile;
     const fd2::GeomDev& g0 = c->hgeom[0];
     if (tile > g0.xdim) tile = g0.xdim;
     if (tile > g0.ydim) tile = g0.ydim;

@@ -29,8 +29,6 @@ from . import events as E
 from . import fd2
 from . import stats
 
-MLP_FWD_FLOP = 2.0 * (5 * 256 + 2 * 256 * 256 + 256 * 65)  # per event
-BF16_PEAK_TFLOPS = 2500.0
 
 
 def _mc(p, ev, seed):
@@ -92,12 +90,6 @@ def evaluate(scale: float = 1.0, net=None, seed: int = 271828182, noise_floor: b
             swank_mc=sw_mc, swank_surrogate=sw_s, swank_rel_err=abs(sw_s / sw_mc - 1.0),
             routing={k: (float(v) if isinstance(v, float) else int(v)) for k, v in st.items()},
         )
-        if st["ms_surrogate"] > 0:
-            # the forward's FLOP over the whole network part (features, GEMMs,
-            # sampling, copies): a lower bound on the GEMMs' own rate
-            out["mlp_forward_tflops"] = (MLP_FWD_FLOP * st["n_surrogate"] /
-                                         (st["ms_surrogate"] * 1e-3) / 1e12)
-            out["mlp_forward_mfma_frac"] = out["mlp_forward_tflops"] / BF16_PEAK_TFLOPS
         if noise_floor:
             _, prf_mc2, hist_mc2, c_mc2, _ = _mc(p, ev, seed + 2)
             out["prf_l1_mc_vs_mc"] = stats.l1_distance(prf_mc2, prf_mc)
