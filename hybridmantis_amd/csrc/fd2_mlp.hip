@@ -17,7 +17,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/fd2.h"
@@ -911,49 +913,103 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
                   q.detradius >= cfg.r_min && q.detradius <= cfg.r_max &&
                   refl >= cfg.refl_min && refl <= cfg.refl_max;
   }
-  // One pass validates and routes.  Monte Carlo and audit events (a few
-  // per cent) are copied; network events are kept as indices into `events`
-  // and gathered on the device.
+  // Validation and routing over contiguous ranges of `events`, on up to 8
+  // host threads.  Monte Carlo and audit events (a few per cent) are
+  // copied; network events are kept as indices into `events` and gathered
+  // on the device.  The ranges' lists joined in order are what one pass in
+  // order would give, and the first range holding an invalid event reports
+  // the first invalid event.
   const int64_t base = c->next_event;
+  const uint64_t audit_cut = (uint64_t)((double)cfg.audit_fraction * 4294967296.0);
+  const uint32_t seed_mix = fd2mlp::mix32(cfg.seed);
+  struct Routed {
+    std::vector<fd2_event> mc, audit;
+    std::vector<uint32_t> idx;
+    int64_t photons_mc = 0, photons_surrogate = 0;
+    int status = FD2_OK;
+  };
+  auto route = [&](int64_t lo, int64_t hi, Routed* r) {
+    r->idx.reserve((size_t)(hi - lo));
+    for (int64_t i = lo; i < hi; ++i) {
+      const fd2_event& e = events[i];
+      if (e.N < 0 || e.N > FD2_MAX_PHOTONS_PER_EVENT) {
+        r->status = FD2_E_TOO_MANY;
+        return;
+      }
+      if (e.geom >= (uint32_t)c->n_geoms) {
+        r->status = FD2_E_INVAL;
+        return;
+      }
+      const fd2::GeomDev& g = c->hgeom[e.geom];
+      const uint32_t gid = (uint32_t)(base + i);
+      const bool near = e.z > g.half_h - cfg.surface_band || e.z < -g.half_h + cfg.surface_band;
+      const float ox = e.x - 0.5f * g.xdet, oy = e.y - 0.5f * g.ydet;
+      const bool off_centre = ox * ox + oy * oy > cfg.center_band * cfg.center_band;
+      const bool off_axis = !(e.col_dx >= 0.0f && e.col_dx < g.R);  // trained on [0, R)
+      if (near || off_centre || off_axis || !in_range[e.geom]) {
+        r->mc.push_back(e);
+        r->photons_mc += e.N;
+      } else if ((uint64_t)fd2mlp::mix32(gid ^ seed_mix) < audit_cut) {
+        r->audit.push_back(e);
+        r->photons_mc += e.N;
+      } else {
+        r->idx.push_back((uint32_t)i);
+        r->photons_surrogate += e.N;
+      }
+    }
+  };
+  const int64_t per_part = (int64_t)1 << 18;
+  int n_parts = (int)std::min<int64_t>(8, (n + per_part - 1) / per_part);
+  const unsigned hw = std::thread::hardware_concurrency();
+  if (hw > 0 && n_parts > (int)hw) n_parts = (int)hw;
+  if (n_parts < 1) n_parts = 1;
+  std::vector<Routed> parts((size_t)n_parts);
+  {
+    std::vector<std::thread> workers;
+    for (int t = 1; t < n_parts; ++t) {
+      const int64_t lo = n * t / n_parts, hi = n * (t + 1) / n_parts;
+      try {
+        workers.emplace_back(route, lo, hi, &parts[t]);
+      } catch (...) {
+        route(lo, hi, &parts[t]);
+      }
+    }
+    route(0, n / n_parts, &parts[0]);
+    for (std::thread& w : workers) w.join();
+  }
   std::vector<fd2_event> mc, audit;
-  std::vector<uint32_t> surr_idx;
-  surr_idx.reserve((size_t)n);
+  int64_t ns = 0;
+  for (const Routed& r : parts) {
+    if (r.status) return r.status;
+    mc.insert(mc.end(), r.mc.begin(), r.mc.end());
+    audit.insert(audit.end(), r.audit.begin(), r.audit.end());
+    ns += (int64_t)r.idx.size();
+    stats.photons_mc += r.photons_mc;
+    stats.photons_surrogate += r.photons_surrogate;
+  }
   // network events per sampling launch: fewer than 2^32 photons (the u32
   // PRF tallies of a launch cannot wrap) and at most 2^30 events (int
   // indexing in the kernel); slice_start holds each launch's first index
   std::vector<int64_t> slice_start;
-  uint64_t slice_photons = 0;
-  const uint64_t audit_cut = (uint64_t)((double)cfg.audit_fraction * 4294967296.0);
-  const uint32_t seed_mix = fd2mlp::mix32(cfg.seed);
-  for (int64_t i = 0; i < n; ++i) {
-    const fd2_event& e = events[i];
-    if (e.N < 0 || e.N > FD2_MAX_PHOTONS_PER_EVENT) return FD2_E_TOO_MANY;
-    if (e.geom >= (uint32_t)c->n_geoms) return FD2_E_INVAL;
-    const fd2::GeomDev& g = c->hgeom[e.geom];
-    const uint32_t gid = (uint32_t)(base + i);
-    const bool near = e.z > g.half_h - cfg.surface_band || e.z < -g.half_h + cfg.surface_band;
-    const float ox = e.x - 0.5f * g.xdet, oy = e.y - 0.5f * g.ydet;
-    const bool off_centre = ox * ox + oy * oy > cfg.center_band * cfg.center_band;
-    const bool off_axis = !(e.col_dx >= 0.0f && e.col_dx < g.R);  // trained on [0, R)
-    if (near || off_centre || off_axis || !in_range[e.geom]) {
-      mc.push_back(e);
-      stats.photons_mc += e.N;
-    } else if ((uint64_t)fd2mlp::mix32(gid ^ seed_mix) < audit_cut) {
-      audit.push_back(e);
-      stats.photons_mc += e.N;
-    } else {
-      if (slice_start.empty() || slice_photons + (uint64_t)e.N > 0xFFFFFFFFull ||
-          (int64_t)surr_idx.size() - slice_start.back() >= ((int64_t)1 << 30)) {
-        slice_start.push_back((int64_t)surr_idx.size());
-        slice_photons = 0;
+  if (ns > 0 && (uint64_t)stats.photons_surrogate <= 0xFFFFFFFFull &&
+      ns <= ((int64_t)1 << 30)) {
+    slice_start.push_back(0);
+  } else if (ns > 0) {
+    uint64_t slice_photons = 0;
+    int64_t k = 0;
+    for (const Routed& r : parts)
+      for (const uint32_t i : r.idx) {
+        const uint64_t N = (uint64_t)events[i].N;
+        if (slice_start.empty() || slice_photons + N > 0xFFFFFFFFull ||
+            k - slice_start.back() >= ((int64_t)1 << 30)) {
+          slice_start.push_back(k);
+          slice_photons = 0;
+        }
+        slice_photons += N;
+        ++k;
       }
-      slice_photons += (uint64_t)e.N;
-      surr_idx.push_back((uint32_t)i);
-      stats.photons_surrogate += e.N;
-    }
   }
   const int64_t n_mc = (int64_t)mc.size(), n_audit = (int64_t)audit.size();
-  const int64_t ns = (int64_t)surr_idx.size();
   stats.n_events = n;
   stats.n_mc = n_mc;
   stats.n_audit = n_audit;
@@ -1021,10 +1077,15 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
     const int dev_cu = c->n_cu > 0 ? c->n_cu : 1;
     if (hipEventRecord(p->e0, p->stream) != hipSuccess ||
         hipMemcpyAsync(d_all, events, (size_t)n * sizeof(fd2_event), hipMemcpyHostToDevice,
-                       p->stream) != hipSuccess ||
-        hipMemcpyAsync(d_idx, surr_idx.data(), (size_t)ns * 4, hipMemcpyHostToDevice,
                        p->stream) != hipSuccess)
       return release(FD2_E_DEVICE);
+    int64_t off = 0;
+    for (const Routed& r : parts) {
+      if (!r.idx.empty() && hipMemcpyAsync(d_idx + off, r.idx.data(), r.idx.size() * 4,
+                                           hipMemcpyHostToDevice, p->stream) != hipSuccess)
+        return release(FD2_E_DEVICE);
+      off += (int64_t)r.idx.size();
+    }
     if (fd2::launch_fill_u32(d_spread, npix0 * fd2mlp::kSpreadCopies, p->stream) != hipSuccess)
       return release(FD2_E_DEVICE);
     hipLaunchKernelGGL(fd2mlp::k_surr_gather, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,

@@ -234,3 +234,29 @@ def test_surrogate_path_is_deterministic(fd2lib):
         assert not np.array_equal(two[0], 2 * one[0])  # new random streams
     finally:
         net.close()
+
+
+def test_surrogate_routing_reports_the_first_invalid_event(fd2lib):
+    """Routing runs over ranges of the events on several host threads; the
+    status is still that of the first invalid event in order, and nothing
+    is tallied."""
+    from hybridmantis_amd import events as E, surrogate as S
+
+    p, ev = E.workload("mono50", scale=0.6)  # ~1.2e6 events: several ranges
+    assert len(ev) > 4 * (1 << 18)
+    net = S.load()
+    try:
+        for first, second, want in (("geom", "N", "invalid argument"),
+                                    ("N", "geom", "more than 900000")):
+            bad = ev.copy()
+            for field, at in ((first, len(ev) // 3), (second, 3 * len(ev) // 4)):
+                if field == "geom":
+                    bad["geom"][at] = 7
+                else:
+                    bad["N"][at] = 900001
+            with fd2lib.Transport(p) as tr:
+                with pytest.raises(fd2lib.Fd2Error, match=want):
+                    fd2lib.surrogate_submit(tr, net, bad)
+                assert tr.counters()["generated"] == 0
+    finally:
+        net.close()
