@@ -1048,7 +1048,8 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
   // Network part, on the network's own stream: its kernels take the compute
   // units that the Monte Carlo launch frees as it drains.  The forward runs
   // in batches into one logits array; one sampling launch then covers all
-  // network events, so that each workgroup flushes its LDS PRF tile once.
+  // network events (one per 2^32 photons), so that each workgroup flushes
+  // its LDS PRF tile once.
   bool timed_net = false;
   if (ns > 0) {
     const int chunk = 65536;
