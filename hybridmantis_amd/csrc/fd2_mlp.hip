@@ -680,12 +680,16 @@ int fd2_mlp_train(fd2_mlp* p, int steps, int B, float lr, uint32_t seed, float* 
 
 namespace fd2mlp {
 
-constexpr int kSampleThreads = 256;
-// side (pixels) of the sampler's LDS tile of geometry 0's PRF, centred on
-// the detector: the sampler runs after the Monte Carlo launch has drained,
-// so its tile is sized for its own occupancy, not the transport's
-constexpr int kSampleTile = 128;
-constexpr int kTileDefault = 96;
+// Sampler shapes: the side (pixels) of the LDS tile of geometry 0's PRF,
+// centred on the detector, and the workgroup size.  The sampler runs after
+// the Monte Carlo launch has drained, so the tile is sized for the
+// sampler's own occupancy, not the transport's.  Large launches take a
+// 176-pixel tile (124 KB, one 1024-thread workgroup per CU); each
+// workgroup flushes its tile once, which small launches would not repay,
+// so they take a 128-pixel tile (64 KB, two 256-thread workgroups per CU).
+constexpr int kSampleTile = 128, kSampleThreads = 256;
+constexpr int kSampleTileLarge = 176, kSampleThreadsLarge = 1024;
+constexpr int kSampleLargeEventsPerWave = 64;  // events per wave for the large shape
 
 // network features of an event (see fd2.h): depth below the top / H,
 // col_dx / R, H / 500 um, R / 5 um, top reflectivity
@@ -758,7 +762,8 @@ constexpr int kSpreadCopies = 8;
 // keeps the central part of geometry 0's PRF in LDS.  The read-only tables
 // come again as __restrict__ parameters, so that the geometry constants are
 // read with scalar loads rather than reloaded around every tally atomic.
-__global__ void __launch_bounds__(kSampleThreads)
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS)
 k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
               const uint32_t* __restrict__ ids, const float* __restrict__ logits,
               const fd2::GeomDev* __restrict__ geoms) {
@@ -769,8 +774,8 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
   for (int i = threadIdx.x; i < a.n_geoms * 3; i += blockDim.x) s_cnt[i] = 0u;
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int waves = gridDim.x * (kSampleThreads / 64);
-  for (int e = blockIdx.x * (kSampleThreads / 64) + w; e < a.n; e += waves) {
+  const int waves = gridDim.x * (THREADS / 64);
+  for (int e = blockIdx.x * (THREADS / 64) + w; e < a.n; e += waves) {
     const fd2_event ev = events[e];
     const fd2::GeomDev& g = geoms[ev.geom];
     const float* z = logits + (size_t)e * kOutPad;
@@ -1064,18 +1069,31 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
     const size_t npix0 = (size_t)c->hgeom[0].xdim * c->hgeom[0].ydim;
     if (hipMalloc(&d_spread, npix0 * fd2mlp::kSpreadCopies * sizeof(uint32_t)) != hipSuccess)
       return release(FD2_E_NOMEM);
-    int tile = fd2mlp::kSampleTile;
     const fd2::GeomDev& g0 = c->hgeom[0];
-    if (tile > g0.xdim) tile = g0.xdim;
-    if (tile > g0.ydim) tile = g0.ydim;
-    const size_t smem = (size_t)tile * tile * 4;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fd2mlp::k_surr_sample,
-                                                     fd2mlp::kSampleThreads, smem) != hipSuccess ||
-        per_cu < 1)
-      return release(FD2_E_DEVICE);
-    if (per_cu > 8) per_cu = 8;
     const int dev_cu = c->n_cu > 0 ? c->n_cu : 1;
+    // a sampler shape: tile side, dynamic LDS, workgroups per CU
+    struct Shape {
+      int tile = 0, per_cu = 0;
+      size_t smem = 0;
+    };
+    auto shape = [&](int side, int threads, const void* kernel, Shape* sh) {
+      int t = side < g0.xdim ? side : g0.xdim;
+      if (t > g0.ydim) t = g0.ydim;
+      sh->tile = t;
+      sh->smem = (size_t)t * t * 4;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&sh->per_cu, kernel, threads, sh->smem) !=
+              hipSuccess ||
+          sh->per_cu < 1)
+        return false;
+      if (sh->per_cu > 8) sh->per_cu = 8;
+      return true;
+    };
+    Shape small, large;
+    if (!shape(fd2mlp::kSampleTile, fd2mlp::kSampleThreads,
+               (const void*)fd2mlp::k_surr_sample<fd2mlp::kSampleThreads>, &small) ||
+        !shape(fd2mlp::kSampleTileLarge, fd2mlp::kSampleThreadsLarge,
+               (const void*)fd2mlp::k_surr_sample<fd2mlp::kSampleThreadsLarge>, &large))
+      return release(FD2_E_DEVICE);
     if (hipEventRecord(p->e0, p->stream) != hipSuccess ||
         hipMemcpyAsync(d_all, events, (size_t)n * sizeof(fd2_event), hipMemcpyHostToDevice,
                        p->stream) != hipSuccess)
@@ -1110,16 +1128,25 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
       a.n = b;
       a.n_geoms = c->n_geoms;
       a.seed = cfg.seed;
-      a.tile = tile;
-      a.tile_i0 = g0.xdim / 2 - tile / 2;
-      a.tile_j0 = g0.ydim / 2 - tile / 2;
+      const int large_waves = dev_cu * large.per_cu * (fd2mlp::kSampleThreadsLarge / 64);
+      const bool big = (int64_t)b >= (int64_t)large_waves * fd2mlp::kSampleLargeEventsPerWave;
+      const Shape& sh = big ? large : small;
+      const int threads = big ? fd2mlp::kSampleThreadsLarge : fd2mlp::kSampleThreads;
+      a.tile = sh.tile;
+      a.tile_i0 = g0.xdim / 2 - sh.tile / 2;
+      a.tile_j0 = g0.ydim / 2 - sh.tile / 2;
       a.spread = d_spread;
-      const int waves_per_block = fd2mlp::kSampleThreads / 64;
-      int64_t blocks = (int64_t)dev_cu * per_cu;
+      const int waves_per_block = threads / 64;
+      int64_t blocks = (int64_t)dev_cu * sh.per_cu;
       if (blocks * waves_per_block > b) blocks = (b + waves_per_block - 1) / waves_per_block;
-      hipLaunchKernelGGL(fd2mlp::k_surr_sample, dim3((unsigned)blocks),
-                         dim3(fd2mlp::kSampleThreads), smem, p->stream, a, a.events, a.ids,
-                         a.logits, (const fd2::GeomDev*)a.geoms);
+      if (big)
+        hipLaunchKernelGGL(fd2mlp::k_surr_sample<fd2mlp::kSampleThreadsLarge>,
+                           dim3((unsigned)blocks), dim3(threads), sh.smem, p->stream, a, a.events,
+                           a.ids, a.logits, (const fd2::GeomDev*)a.geoms);
+      else
+        hipLaunchKernelGGL(fd2mlp::k_surr_sample<fd2mlp::kSampleThreads>, dim3((unsigned)blocks),
+                           dim3(threads), sh.smem, p->stream, a, a.events, a.ids, a.logits,
+                           (const fd2::GeomDev*)a.geoms);
       hipLaunchKernelGGL(fd2mlp::k_spread_flush, dim3((unsigned)((npix0 + 255) / 256)), dim3(256),
                          0, p->stream, d_spread, npix0, c->d_prf[0]);
       if (hipGetLastError() != hipSuccess) rc = FD2_E_DEVICE;

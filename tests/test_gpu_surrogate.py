@@ -260,3 +260,29 @@ def test_surrogate_routing_reports_the_first_invalid_event(fd2lib):
                 assert tr.counters()["generated"] == 0
     finally:
         net.close()
+
+
+def test_surrogate_sampler_shapes_agree(fd2lib):
+    """Large calls take the 1024-thread sampler with a 176-pixel LDS tile,
+    small ones the 256-thread sampler with a 128-pixel tile.  Each event's
+    draws are keyed by its event number, so one large call and the same
+    events in four small calls on one context give the same tallies."""
+    from hybridmantis_amd import events as E, surrogate as S
+
+    p, ev = E.workload("mono50", scale=0.2)  # ~4e5 network events: the large shape
+    # the large shape from 64 events per wave of 256 CUs x 16 waves
+    assert len(ev) > 256 * 16 * 64 > len(ev) // 4
+    cfg = dict(surface_band=0.0, center_band=1e6, audit_fraction=0.0)
+    net = S.load()
+    try:
+        with fd2lib.Transport(p) as tr:
+            st = fd2lib.surrogate_submit(tr, net, ev, **cfg)
+            one = _tallies(tr)
+        assert st["n_surrogate"] == len(ev)
+        with fd2lib.Transport(p) as tr:
+            for part in np.array_split(ev, 4):
+                fd2lib.surrogate_submit(tr, net, part, **cfg)
+            four = _tallies(tr)
+        _same(one, four)
+    finally:
+        net.close()
