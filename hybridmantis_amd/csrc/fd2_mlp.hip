@@ -1027,13 +1027,18 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
   uint32_t* d_rad = nullptr;
   float* d_logits = nullptr;
   uint32_t* d_spread = nullptr;
+  float* d_feat = nullptr;  // audit features and predictions
+  float* d_pred = nullptr;
+  float* h_pred = nullptr;  // pinned
+  std::vector<float> feat;
   // waits for both streams before the buffers go
   auto release = [&](int status) {
     if (hipStreamSynchronize(c->stream) != hipSuccess && !status) status = FD2_E_DEVICE;
     if (hipStreamSynchronize(p->stream) != hipSuccess && !status) status = FD2_E_DEVICE;
     for (void* b : {(void*)d_all, (void*)d_ev, (void*)d_idx, (void*)d_ids, (void*)d_rad,
-                    (void*)d_logits, (void*)d_spread})
+                    (void*)d_logits, (void*)d_spread, (void*)d_feat, (void*)d_pred})
       if (b) (void)hipFree(b);
+    if (h_pred) (void)hipHostFree(h_pred);
     return status;
   };
 
@@ -1048,6 +1053,46 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
       return release(FD2_E_NOMEM);
     rc = fd2::submit_radial_async(c, mc.data(), (int64_t)mc.size(), d_rad);
     if (rc) return release(rc);
+  }
+
+  // The network's prediction for the audit events needs no Monte Carlo
+  // result: it goes first on the network's stream, with its copy to pinned
+  // host memory, so that the end of the call only reads it.
+  if (n_audit > 0) {
+    const int chunk = 65536;
+    rc = ensure_batch(p, chunk);
+    if (rc) return release(rc);
+    feat.resize((size_t)n_audit * kFeat);
+    for (int64_t i = 0; i < n_audit; ++i) {
+      const fd2_event& e = audit[i];
+      const fd2::GeomDev& g = c->hgeom[e.geom];
+      feat[i * kFeat + 0] = (g.half_h - e.z) / g.H;
+      feat[i * kFeat + 1] = e.col_dx / g.R;
+      feat[i * kFeat + 2] = g.H * (1.0f / 500.0f);
+      feat[i * kFeat + 3] = g.R * (1.0f / 5.0f);
+      feat[i * kFeat + 4] = 1.0f - g.top_absfrac;
+    }
+    const size_t pred_bytes = (size_t)n_audit * kTarget * sizeof(float);
+    if (hipMalloc(&d_feat, feat.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d_pred, pred_bytes) != hipSuccess ||
+        hipHostMalloc(&h_pred, pred_bytes, hipHostMallocDefault) != hipSuccess)
+      return release(FD2_E_NOMEM);
+    if (hipMemcpyAsync(d_feat, feat.data(), feat.size() * sizeof(float), hipMemcpyHostToDevice,
+                       p->stream) != hipSuccess)
+      return release(FD2_E_DEVICE);
+    for (int64_t s0 = 0; s0 < n_audit && rc == FD2_OK; s0 += chunk) {
+      const int b = (int)(n_audit - s0 < chunk ? n_audit - s0 : chunk);
+      hipLaunchKernelGGL(k_pack_inputs, dim3((b * kIn + 255) / 256), dim3(256), 0, p->stream,
+                         d_feat + (size_t)s0 * kFeat, b, p->x0);
+      rc = forward(p, b, p->stream);
+      if (rc) break;
+      hipLaunchKernelGGL(k_head_probs, dim3((b * 64 + 255) / 256), dim3(256), 0, p->stream,
+                         p->z[3], b, d_pred + (size_t)s0 * kTarget);
+    }
+    if (rc) return release(rc);
+    if (hipMemcpyAsync(h_pred, d_pred, pred_bytes, hipMemcpyDeviceToHost, p->stream) !=
+        hipSuccess)
+      return release(FD2_E_DEVICE);
   }
 
   // Network part, on the network's own stream: its kernels take the compute
@@ -1168,21 +1213,10 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
       rc = FD2_E_DEVICE;
     if (rc) return release(rc);
   }
-  // the network's prediction for the audit events (queued behind the
-  // network part on the network's stream)
+  // the audit comparison, once the network's stream is done
   if (n_audit > 0) {
-    std::vector<float> feat((size_t)n_audit * kFeat), pred((size_t)n_audit * kTarget);
-    for (int64_t i = 0; i < n_audit; ++i) {
-      const fd2_event& e = audit[i];
-      const fd2::GeomDev& g = c->hgeom[e.geom];
-      feat[i * kFeat + 0] = (g.half_h - e.z) / g.H;
-      feat[i * kFeat + 1] = e.col_dx / g.R;
-      feat[i * kFeat + 2] = g.H * (1.0f / 500.0f);
-      feat[i * kFeat + 3] = g.R * (1.0f / 5.0f);
-      feat[i * kFeat + 4] = 1.0f - g.top_absfrac;
-    }
-    rc = fd2_mlp_forward(p, feat.data(), n_audit, pred.data());
-    if (rc) return release(rc);
+    if (hipStreamSynchronize(p->stream) != hipSuccess) return release(FD2_E_DEVICE);
+    const float* pred = h_pred;
     std::vector<double> pm(FD2_RADIAL_BINS, 0.0), pp(FD2_RADIAL_BINS, 0.0);
     for (int64_t i = 0; i < n_audit; ++i) {
       double det = 0.0;
