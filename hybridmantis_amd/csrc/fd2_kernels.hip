@@ -504,11 +504,19 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
           pool_done = true;
           break;
         }
-        // largest e with chunk_off[e] <= c (skips events without photons)
+        // largest e with chunk_off[e] <= c (skips events without photons):
+        // each round the 64 lanes (all active here) probe 64 points of
+        // [lo, hi) with one vector load, and the wave keeps the step after
+        // the last probe at or below c -- 3 rounds for 2e5 events where a
+        // binary search makes 18 dependent loads
         int64_t lo = 0, hi = a.n_events;
         while (hi - lo > 1) {
-          const int64_t mid = (lo + hi) >> 1;
-          if (chunk_off[mid] <= c) lo = mid; else hi = mid;
+          const int64_t step = (hi - lo + 63) >> 6;
+          const int64_t pos = lo + (int64_t)lane * step;
+          const bool le = pos < hi && chunk_off[pos] <= c;
+          const unsigned long long m = __ballot(le);
+          lo += (int64_t)(63 - __clzll(m)) * step;
+          hi = min(hi, lo + step);
         }
         pool_ev = lo;
         const int32_t n_ph = events[lo].N;
