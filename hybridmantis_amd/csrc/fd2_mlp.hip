@@ -879,9 +879,8 @@ uint32_t mix32(uint32_t x) {  // integer hash (murmur3 finaliser)
 
 }  // namespace fd2mlp
 
-extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events,
-                                    int64_t n, const fd2_surrogate_cfg* cfg_in,
-                                    fd2_surrogate_stats* st) {
+static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int64_t n,
+                            const fd2_surrogate_cfg* cfg_in, fd2_surrogate_stats* st) {
   if (!c || !p || n < 0 || (n > 0 && !events)) return FD2_E_INVAL;
   if (c->device != p->device) return FD2_E_INVAL;
   if (n > (int64_t)UINT32_MAX) return FD2_E_INVAL;  // network events are routed by u32 index
@@ -933,7 +932,7 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
     int64_t photons_mc = 0, photons_surrogate = 0;
     int status = FD2_OK;
   };
-  auto route = [&](int64_t lo, int64_t hi, Routed* r) {
+  auto route_range = [&](int64_t lo, int64_t hi, Routed* r) {
     r->idx.reserve((size_t)(hi - lo));
     for (int64_t i = lo; i < hi; ++i) {
       const fd2_event& e = events[i];
@@ -963,6 +962,14 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
       }
     }
   };
+  // an allocation failure ends the range with a status, also on a worker thread
+  auto route = [&](int64_t lo, int64_t hi, Routed* r) {
+    try {
+      route_range(lo, hi, r);
+    } catch (...) {
+      r->status = FD2_E_NOMEM;
+    }
+  };
   const int64_t per_part = (int64_t)1 << 18;
   int n_parts = (int)std::min<int64_t>(8, (n + per_part - 1) / per_part);
   const unsigned hw = std::thread::hardware_concurrency();
@@ -971,6 +978,10 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
   std::vector<Routed> parts((size_t)n_parts);
   {
     std::vector<std::thread> workers;
+    try {
+      workers.reserve((size_t)n_parts);
+    } catch (...) {
+    }
     for (int t = 1; t < n_parts; ++t) {
       const int64_t lo = n * t / n_parts, hi = n * (t + 1) / n_parts;
       try {
@@ -1249,4 +1260,15 @@ extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* eve
   stats.ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (st) *st = stats;
   return FD2_OK;
+}
+
+// host allocation failures come back as a status, not an exception
+extern "C" int fd2_surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events,
+                                    int64_t n, const fd2_surrogate_cfg* cfg_in,
+                                    fd2_surrogate_stats* st) {
+  try {
+    return surrogate_submit(c, p, events, n, cfg_in, st);
+  } catch (...) {
+    return FD2_E_NOMEM;
+  }
 }
