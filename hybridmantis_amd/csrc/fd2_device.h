@@ -16,9 +16,29 @@
 #include <stdint.h>
 
 #include "../../include/fd2.h"
+#include "fd2_ff.h"
 #include "fd2_math.h"
 
 namespace fd2 {
+
+// Arithmetic policies of the transport code below.
+//   MathDevice    the production kernel's FP32 arithmetic (fd2_math.h):
+//                 azimuths in turns, reciprocal square roots, the Fresnel
+//                 branch from cosines and sines (Snell's law).
+//   MathFaithful  the reference C path's own arithmetic: float state, each
+//                 double <math.h> expression evaluated with float pairs
+//                 (fd2_ff.h) and rounded to float as the reference rounds
+//                 it, the C library's acosf/asinf restated, the angles
+//                 themselves in the Fresnel branch.  Used by the trajectory
+//                 test's kernel only; no FP64 instruction either way.
+// The control flow, draw order and every float operation the reference
+// writes in float are the same under both.
+struct MathDevice {
+  static constexpr bool kFaithful = false;
+};
+struct MathFaithful {
+  static constexpr bool kFaithful = true;
+};
 
 // The reference's azimuths are u * twopipen (main/kernel_cuda_c_ver1_0.cu:56);
 // here they are the turn u itself (fd2_sincos_turns).
@@ -53,6 +73,9 @@ struct GeomDev {
   int32_t gap_fresnel; // (n1/n2) >= 1.57: gap side recomputes Pr/Pt
   float inv_pixel;     // 1/pixel when pixel is a power of two (x/pixel ==
                        // x*inv_pixel exactly), else 0
+  // MathFaithful: total internal reflection as theta1 > the largest float
+  // <= asin(n2/n1) (inside) or asin(n1/n2) (gap; NaN, never, when n1 > n2)
+  float t1_tir_in, t1_tir_gap;
   unsigned long long* prf;   // xdim*ydim
   uint32_t* hist;            // num_hist
 };
@@ -73,13 +96,15 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0,
 }
 
 // Production stream: xoshiro128** (Blackman & Vigna 2018) per photon, its
-// 128-bit state = Philox4x32-10(counter (photon,0,0,0), key (seed, event)).
+// 128-bit state = Philox4x32-10(counter (photon, event >> 32, 0, 0),
+// key (seed, event mod 2^32)): distinct for all 2^64 event numbers, and the
+// same as a 32-bit event key below 2^32.
 struct CounterRng {
   uint32_t s0, s1, s2, s3;
-  __device__ __forceinline__ void seed(uint32_t key0, uint32_t event,
+  __device__ __forceinline__ void seed(uint32_t key0, uint64_t event,
                                        uint32_t photon) {
-    uint32_t c[4] = {photon, 0u, 0u, 0u};
-    philox4x32_10(c, key0, event);
+    uint32_t c[4] = {photon, (uint32_t)(event >> 32), 0u, 0u};
+    philox4x32_10(c, key0, (uint32_t)event);
     s0 = c[0]; s1 = c[1]; s2 = c[2]; s3 = c[3];
     if ((s0 | s1 | s2 | s3) == 0u) s0 = 0x9E3779B9u;
   }
@@ -209,24 +234,35 @@ __device__ __forceinline__ int32_t pix_index(int32_t pix, const GeomDev& g) {
 
 // ------------------------------------------------------- vector helpers
 
+template <class M = MathDevice>
 __device__ __forceinline__ void renorm(float& x, float& y, float& z,
                                        const GeomDev& g) {
-  float nrm = fd2_sqrt_unit(x * x + y * y + z * z);
-  if (nrm < g.norm_lo || nrm > g.norm_hi) {
-    const float inv = 1.0f / nrm;
-    x = x * inv;
-    y = y * inv;
-    z = z * inv;
+  if constexpr (M::kFaithful) {  // (float)sqrt((double)s) is sqrtf(s)
+    const float nrm = sqrtf(x * x + y * y + z * z);
+    if (nrm < g.norm_lo || nrm > g.norm_hi) {
+      x = x / nrm;
+      y = y / nrm;
+      z = z / nrm;
+    }
+  } else {
+    float nrm = fd2_sqrt_unit(x * x + y * y + z * z);
+    if (nrm < g.norm_lo || nrm > g.norm_hi) {
+      const float inv = 1.0f / nrm;
+      x = x * inv;
+      y = y * inv;
+      z = z * inv;
+    }
   }
 }
 
 // trans_dir_cos (main/kernel_cuda_c_ver1_0.cu:3169-3205)
+template <class M = MathDevice>
 __device__ __forceinline__ void reflect(Photon& p, const GeomDev& g) {
   float ca = -p.ux * p.nx + -p.uy * p.ny + -p.uz * p.nz;
   p.ux = 2.0f * ca * p.nx + p.ux;
   p.uy = 2.0f * ca * p.ny + p.uy;
   p.uz = 2.0f * ca * p.nz + p.uz;
-  renorm(p.ux, p.uy, p.uz, g);
+  renorm<M>(p.ux, p.uy, p.uz, g);
 }
 
 // refraction with sin(theta1) = s1, sin(theta2) = s2, cos(theta2) = c2
@@ -240,65 +276,112 @@ __device__ __forceinline__ void refract(Photon& p, float s1, float s2, float c2,
   renorm(p.ux, p.uy, p.uz, g);
 }
 
+// refraction from the angles theta1, theta2, MathFaithful:
+// d' = -n cos(t2) - (sin t2 / sin t1)(d + (-d.n) n), each component a double
+// expression rounded to float
+__device__ __forceinline__ void refract_angles(Photon& p, float t1, float t2,
+                                               const GeomDev& g) {
+  const float ca = -p.ux * p.nx + -p.uy * p.ny + -p.uz * p.nz;
+  const fd2_ff ct = fd2x_cos(t2);
+  const fd2_ff ratio = fd2x_div(fd2x_sin(t2), fd2x_sin(t1));
+  p.ux = fd2x_refract_component(p.nx, p.ux + ca * p.nx, ct, ratio);
+  p.uy = fd2x_refract_component(p.ny, p.uy + ca * p.ny, ct, ratio);
+  p.uz = fd2x_refract_component(p.nz, p.uz + ca * p.nz, ct, ratio);
+  renorm<MathFaithful>(p.ux, p.uy, p.uz, g);
+}
+
+// x, y direction cosines rr (cos, sin) of the azimuth 2 pi u: in turns
+// (MathDevice), or theta = (float)(u * twopipen) and the double products
+// (MathFaithful, main/kernel_cuda_c_ver1_0.cu:582-584)
+template <class M>
+__device__ __forceinline__ void azimuth(float rr, float u, float& x, float& y) {
+  if constexpr (M::kFaithful) {
+    fd2x_polar(rr, fd2x_mul_dconst(u, FD2X_TWOPI_D), &x, &y);
+  } else {
+    float sn, cs;
+    fd2_sincos_turns(u, &sn, &cs);
+    x = rr * cs;
+    y = rr * sn;
+  }
+}
+
 // RoughSurface (main/kernel_cuda_c_ver1_0.cu:3261-3300): perturbation with
 // z = 2s-1 and radius sqrt(1-s^2) (s, not z), normalised, then
 // n <- normalise(beta*p + n).  rough_offset draws the normalised p.
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ void rough_offset(Rng& rng, float& qx, float& qy, float& qz) {
   float s = rng.u();
   qz = 2.0f * s - 1.0f;
-  float rr = fd2_sqrt_unit(1.0f - s * s);
-  s = rng.u();
-  float sn, cs;
-  fd2_sincos_turns(s, &sn, &cs);  // azimuth 2 pi s
-  qx = rr * cs;
-  qy = rr * sn;
-  float inv = fd2_rsqrt(qx * qx + qy * qy + qz * qz);
-  qx = qx * inv;
-  qy = qy * inv;
-  qz = qz * inv;
+  if constexpr (M::kFaithful) {
+    // theta = s * 2.0f * pi with the double pi (K:3275), the double
+    // cos/sin products and sqrt(pow + pow + pow) (K:3277-3285)
+    const float rr = sqrtf(1.0f - s * s);
+    s = rng.u();
+    fd2x_polar(rr, fd2x_mul_dconst(s * 2.0f, FD2X_PI_D), &qx, &qy);
+    const float nb = fd2x_norm3(qx, qy, qz);
+    qx = qx / nb;
+    qy = qy / nb;
+    qz = qz / nb;
+  } else {
+    float rr = fd2_sqrt_unit(1.0f - s * s);
+    s = rng.u();
+    float sn, cs;
+    fd2_sincos_turns(s, &sn, &cs);  // azimuth 2 pi s
+    qx = rr * cs;
+    qy = rr * sn;
+    float inv = fd2_rsqrt(qx * qx + qy * qy + qz * qz);
+    qx = qx * inv;
+    qy = qy * inv;
+    qz = qz * inv;
+  }
 }
 
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ void rough(Rng& rng, float& nx, float& ny, float& nz,
                                       float beta) {
   float qx, qy, qz;
-  rough_offset(rng, qx, qy, qz);
+  rough_offset<Rng, M>(rng, qx, qy, qz);
   float rx = beta * qx + nx, ry = beta * qy + ny, rz = beta * qz + nz;
-  float inv = fd2_rsqrt(rx * rx + ry * ry + rz * rz);
-  nx = rx * inv;
-  ny = ry * inv;
-  nz = rz * inv;
+  if constexpr (M::kFaithful) {
+    const float nb = fd2x_norm3(rx, ry, rz);
+    nx = rx / nb;
+    ny = ry / nb;
+    nz = rz / nb;
+  } else {
+    float inv = fd2_rsqrt(rx * rx + ry * ry + rz * rz);
+    nx = rx * inv;
+    ny = ry * inv;
+    nz = rz * inv;
+  }
 }
 
 // isotropic direction at photon termination / trapped restart
 // (main/kernel_cuda_c_ver1_0.cu:578-592)
-template <class Rng>
+// (MathFaithful: the radius is sqrt(1.0 - r*r) in double, K:581)
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ void emit_dir(Rng& rng, Photon& p, const GeomDev& g) {
   float r = rng.u() * 2.0f - 1.0f;
   p.uz = r;
-  float rr = fd2_sqrt_unit(1.0f - r * r);
-  float sn, cs;
-  fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
-  p.ux = rr * cs;
-  p.uy = rr * sn;
-  renorm(p.ux, p.uy, p.uz, g);
+  float rr;
+  if constexpr (M::kFaithful)
+    rr = fd2x_sqrt_one_minus(r * r);
+  else
+    rr = fd2_sqrt_unit(1.0f - r * r);
+  azimuth<M>(rr, rng.u(), p.ux, p.uy);
+  renorm<M>(p.ux, p.uy, p.uz, g);
 }
 
 // first photon of an event: |2u-1| > 0.01 by rejection
 // (main/hybridMANTIS_c_ver1_0.c:257-277)
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ void emit_first_dir(Rng& rng, Photon& p,
                                                const GeomDev& g) {
   float r = rng.u() * 2.0f - 1.0f;
   while (fabsf(r) <= 0.01f) r = rng.u() * 2.0f - 1.0f;
   p.uz = r;
   float rr = fd2_sqrt_unit(1.0f - r * r);
-  float sn, cs;
-  fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
-  p.ux = rr * cs;
-  p.uy = rr * sn;
-  renorm(p.ux, p.uy, p.uz, g);
+  azimuth<M>(rr, rng.u(), p.ux, p.uy);
+  renorm<M>(p.ux, p.uy, p.uz, g);
 }
 
 // ---------------------------------------------------------- transport
@@ -323,7 +406,7 @@ struct StepSink {
 // z = +-H/2; lost check; out-of-column nudge (back 0.5 um along -d, halving
 // on overshoot, at most 10+10 tries).  Returns the distance, or -99 when the
 // photon was lost or left its column (fate set).
-template <bool TRACE>
+template <bool TRACE, class M = MathDevice>
 __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
                                                  StepSink* sink) {
   const float H2 = g.half_h, R = g.R;
@@ -333,10 +416,15 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
   const float qa = p.ux * p.ux + p.uy * p.uy;
   const float qb = 2.0f * (dx0 * p.ux + dy0 * p.uy);
   const float qc = dx0 * dx0 + dy0 * dy0 - R * R;
-  const float sq = sqrtf(qb * qb - 4.0f * qa * qc);
-  const float inv2a = 1.0f / (2.0f * qa);
-  const float r1 = (-qb + sq) * inv2a;
-  const float r2 = (-qb - sq) * inv2a;
+  float r1, r2;
+  if constexpr (M::kFaithful) {  // double root and quotients (K:992-998)
+    fd2x_roots(qb, qb * qb - 4.0f * qa * qc, 2.0f * qa, &r1, &r2);
+  } else {
+    const float sq = sqrtf(qb * qb - 4.0f * qa * qc);
+    const float inv2a = 1.0f / (2.0f * qa);
+    r1 = (-qb + sq) * inv2a;
+    r2 = (-qb - sq) * inv2a;
+  }
   float dcyl = 999.0f;
   if (r1 >= r2)
     dcyl = r1;
@@ -422,12 +510,14 @@ __device__ __forceinline__ float dist_to_surface(Photon& p, const GeomDev& g,
 // isotropic (main/kernel_cuda_c_ver1_0.cu:702-732): in-column step, then
 // bulk absorption at -ln(u)/mu (drawn even when the step was lost).
 // true = photon ended.
-template <class Rng, bool TRACE>
+template <class Rng, bool TRACE, class M = MathDevice>
 __device__ __forceinline__ bool isotropic(Rng& rng, Photon& p, const GeomDev& g,
                                           StepSink* sink) {
-  const float ds = dist_to_surface<TRACE>(p, g, sink);
+  const float ds = dist_to_surface<TRACE, M>(p, g, sink);
   float da = 999999.0f;
-  if (g.bulk > 0.0f) {
+  if (M::kFaithful && g.bulk > 0.0f) {  // (float)((double)(-1/mu) * log(u)), K:712
+    da = fd2x_scaled_log(g.neg_inv_bulk, rng.u());
+  } else if (g.bulk > 0.0f) {
     const float u = rng.u();
     // -ln u >= 1 - u, so the absorption length da = -ln(u)/bulk exceeds a
     // step shorter than (1 - u)/bulk less a 2^-20 margin (which covers the
@@ -456,14 +546,14 @@ __device__ __forceinline__ bool isotropic(Rng& rng, Photon& p, const GeomDev& g,
 // once.
 enum SensorResult : int { kSensorNone = 0, kSensorDetect = 1, kSensorReflect = 2 };
 
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ int detection(Rng& rng, Photon& p, const GeomDev& g) {
   if (fabsf(p.pz - (-g.half_h)) < g.eps_lt) {
     if (rng.u() < g.sensor_refl) {
       p.nx = 0.0f;
       p.ny = 0.0f;
       p.nz = 1.0f;
-      reflect(p, g);
+      reflect<M>(p, g);
       return kSensorReflect;
     }
     tally_detect(p, g);
@@ -482,39 +572,64 @@ __device__ __forceinline__ bool angle_over_157(float x) {
   return (x >= -1.0f && x <= 7.90e-4f) || (x > 7.90e-4f && x < 8.03e-4f && near > 1.57f);
 }
 
+// acos(x) < 0 || acos(x) > 1.57 under either arithmetic (MathFaithful: the C
+// library's acosf, K:2531, :1904)
+template <class M>
+__device__ __forceinline__ bool angle_over(float x) {
+  if constexpr (M::kFaithful) {
+    const float a = fd2x_acosf(x);
+    return a < 0.0f || a > 1.57f;
+  } else {
+    return angle_over_157(x);
+  }
+}
+
 // New column orientation (transmit/boundary_analysis/refl_bottom,
 // main/kernel_cuda_c_ver1_0.cu:2452-2488, 2501-2538, 1769-1804, 3036-3070):
 // n = -d perturbed with beta = 1, projected on xy and renormalised, then
 // perturbed with beta (always for refl_bottom); accept when
 // acos(-d.n) <= 1.57.  One sample; true = accepted (p.n is the sample
 // either way).
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ bool column_normal_sample(Rng& rng, Photon& p, const GeomDev& g,
                                                      float mx, float my, float mz,
                                                      bool always_rough) {
-  // -d perturbed with beta = 1 and projected on xy: the xy part of p + m is
-  // normalised directly (the same direction as normalising p + m first)
-  float qx, qy, qz;
-  rough_offset(rng, qx, qy, qz);
-  float nx = qx + mx, ny = qy + my, nz = 0.0f;
-  const float inv = fd2_rsqrt(nx * nx + ny * ny);
-  nx = nx * inv;
-  ny = ny * inv;
-  if (always_rough || g.beta > 0.0f) rough(rng, nx, ny, nz, g.beta);
+  float nx, ny, nz = 0.0f;
+  if constexpr (M::kFaithful) {
+    // RoughSurface(beta = 1) of -d, then the xy projection renormalised
+    nx = mx;
+    ny = my;
+    float tz = mz;
+    rough<Rng, M>(rng, nx, ny, tz, 1.0f);
+    const float den = sqrtf(nx * nx + ny * ny);
+    nx = nx / den;
+    ny = ny / den;
+  } else {
+    // -d perturbed with beta = 1 and projected on xy: the xy part of p + m
+    // is normalised directly (the same direction as normalising p + m first)
+    float qx, qy, qz;
+    rough_offset(rng, qx, qy, qz);
+    nx = qx + mx;
+    ny = qy + my;
+    const float inv = fd2_rsqrt(nx * nx + ny * ny);
+    nx = nx * inv;
+    ny = ny * inv;
+  }
+  if (always_rough || g.beta > 0.0f) rough<Rng, M>(rng, nx, ny, nz, g.beta);
   p.nx = nx;
   p.ny = ny;
   p.nz = nz;
-  return !angle_over_157(mx * nx + my * ny + mz * nz);
+  return !angle_over<M>(mx * nx + my * ny + mz * nz);
 }
 
 // Samples until accepted, while the caller's counter < 100.  true = photon
 // killed (theta1).
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ bool column_normal(Rng& rng, Photon& p,
                                               const GeomDev& g, float mx,
                                               float my, float mz, int& tries,
                                               bool always_rough) {
-  while (!column_normal_sample(rng, p, g, mx, my, mz, always_rough)) {
+  while (!column_normal_sample<Rng, M>(rng, p, g, mx, my, mz, always_rough)) {
     if (tries >= 100) {
       p.fate = kTheta1;
       return true;
@@ -533,7 +648,7 @@ __device__ __forceinline__ bool outside_xy(const Photon& p, const GeomDev& g) {
 }
 
 // refl_bottom (main/kernel_cuda_c_ver1_0.cu:2985-3087).  true = ended.
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ bool refl_bottom(Rng& rng, Photon& p,
                                             const GeomDev& g) {
   const float tx = p.px, ty = p.py, tz = p.pz;
@@ -558,19 +673,19 @@ __device__ __forceinline__ bool refl_bottom(Rng& rng, Photon& p,
     return true;
   }
   int tries = 0;
-  return column_normal(rng, p, g, mx, my, mz, tries, true);
+  return column_normal<Rng, M>(rng, p, g, mx, my, mz, tries, true);
 }
 
 // sensor plane reached between columns: reflect + refl_bottom, or detect.
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ bool gap_sensor(Rng& rng, Photon& p,
                                            const GeomDev& g) {
   if (rng.u() < g.sensor_refl) {
     p.nx = 0.0f;
     p.ny = 0.0f;
     p.nz = 1.0f;
-    reflect(p, g);
-    return refl_bottom(rng, p, g);
+    reflect<M>(p, g);
+    return refl_bottom<Rng, M>(rng, p, g);
   }
   tally_detect(p, g);
   return true;
@@ -580,7 +695,7 @@ __device__ __forceinline__ bool gap_sensor(Rng& rng, Photon& p,
 // flight of U(dmin,dmax) between columns against the detector's top
 // (reflector/absorber), bottom (sensor) and sides, then the next column's
 // orientation.  true = ended.
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ bool transmit_gap(Rng& rng, Photon& p,
                                           const GeomDev& g) {
   const float H2 = g.half_h;
@@ -606,14 +721,14 @@ __device__ __forceinline__ bool transmit_gap(Rng& rng, Photon& p,
       p.px = tx + p.ux * dbot;
       p.py = ty + p.uy * dbot;
       p.pz = -H2;
-      if (gap_sensor(rng, p, g)) return true;
+      if (gap_sensor<Rng, M>(rng, p, g)) return true;
       if (at_top_up(p, g)) goto top;
     } else {
       p.fate = kLost;
       return true;
     }
   }
-  if (column_normal(rng, p, g, mx, my, mz, tries1, false)) return true;
+  if (column_normal<Rng, M>(rng, p, g, mx, my, mz, tries1, false)) return true;
   if (fabsf(p.px) < g.eps_lt || fabsf(p.px - g.xdet) < g.eps_lt ||
       fabsf(p.py) < g.eps_lt || fabsf(p.py - g.ydet) < g.eps_lt) {
     p.fate = kLost;
@@ -633,13 +748,7 @@ top:
       return true;
     }
     p.uz = -fabsf(rng.u() * 2.0f - 1.0f);
-    {
-      const float rr = fd2_sqrt_unit(1.0f - p.uz * p.uz);
-      float sn, cs;
-      fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
-      p.ux = rr * cs;
-      p.uy = rr * sn;
-    }
+    azimuth<M>(fd2_sqrt_unit(1.0f - p.uz * p.uz), rng.u(), p.ux, p.uy);
     tx = p.px;
     ty = p.py;
     tz = p.pz;
@@ -660,17 +769,17 @@ top:
         p.px = tx + p.ux * dbot;
         p.py = ty + p.uy * dbot;
         p.pz = -H2;
-        if (gap_sensor(rng, p, g)) return true;
+        if (gap_sensor<Rng, M>(rng, p, g)) return true;
         if (at_top_up(p, g)) goto top;
       } else {
         p.fate = kLost;
         return true;
       }
     }
-    if (column_normal(rng, p, g, mx, my, mz, tries2, false)) return true;
+    if (column_normal<Rng, M>(rng, p, g, mx, my, mz, tries2, false)) return true;
   }
   if (fabsf(p.pz - (-H2)) < g.eps_lt) {
-    if (gap_sensor(rng, p, g)) return true;
+    if (gap_sensor<Rng, M>(rng, p, g)) return true;
     if (at_top_up(p, g)) goto top;
   }
   return false;
@@ -686,7 +795,7 @@ top:
 // stream go with it.
 enum BaResult : int { kBaAlive = 0, kBaEnded = 1, kBaPrpt = 2, kBaCross = 3 };
 
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
   const float H2 = g.half_h;
   // top reflector reached inside a column (K:1610-1634)
@@ -699,11 +808,7 @@ __device__ __forceinline__ int ba_entry(Rng& rng, Photon& p, const GeomDev& g) {
     p.ny = 0.0f;
     p.nz = -1.0f;
     p.uz = -fabsf(rng.u() * 2.0f - 1.0f);
-    const float rr = fd2_sqrt_unit(1.0f - p.uz * p.uz);
-    float sn, cs;
-    fd2_sincos_turns(rng.u(), &sn, &cs);  // azimuth 2 pi u
-    p.ux = rr * cs;
-    p.uy = rr * sn;
+    azimuth<M>(fd2_sqrt_unit(1.0f - p.uz * p.uz), rng.u(), p.ux, p.uy);
     return kBaAlive;
   }
   // crosstalk probability, piecewise linear in depth (K:1639-1652)
@@ -749,18 +854,25 @@ struct PrptState {
 __device__ __forceinline__ PrptState prpt_initial() { return PrptState{0u, 0.0f, 0.0f, 1.0f}; }
 enum PrptResult : int { kPrAlive = 0, kPrEnded = 1, kPrGap = 2 };
 
-template <class Rng>
+//
+// MathFaithful keeps the reference's angles instead: theta1 = acosf of the
+// dot product, theta2 from asinf, Pr and Pt from the angle sums and
+// differences, normalised by their sum on every pass (stale values
+// included, as the reference does).  Its state is local: that arithmetic
+// runs only in the whole-branch form (ba_prpt), never with stop_at_gap.
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ int prpt_loop(Rng& rng, Photon& p, const GeomDev& g, PrptState& st,
                                          bool inside, bool stop_at_gap) {
   const float R = g.R;
   int theta_tries = (int)(st.tries & 0xffu), smooth_tries = (int)((st.tries >> 8) & 0xffu);
   int reperturb = (int)((st.tries >> 16) & 0xffu), normal_tries = (int)(st.tries >> 24);
   float pr = st.pr, s2 = st.s2, c2 = st.c2, s1 = 0.0f;
+  float pt = 0.0f, t1 = 0.0f, t2 = 0.0f;  // MathFaithful: Pt, theta1, theta2
   float onx, ony, onz, odx, ody, odz;
   float mx, my, mz;
 
   if (!inside) {
-    if (transmit_gap(rng, p, g)) return kPrEnded;
+    if (transmit_gap<Rng, M>(rng, p, g)) return kPrEnded;
   }
 
 prpt:
@@ -770,11 +882,17 @@ prpt:
   }
   if (inside) {
     const float ax = p.xc - p.px, ay = p.yc - p.py;
-    const float inv = fd2_rsqrt(ax * ax + ay * ay);
-    p.nx = ax * inv;
-    p.ny = ay * inv;
+    if constexpr (M::kFaithful) {  // K:1686-1690
+      const float mag = sqrtf(ax * ax + ay * ay);
+      p.nx = ax / mag;
+      p.ny = ay / mag;
+    } else {
+      const float inv = fd2_rsqrt(ax * ax + ay * ay);
+      p.nx = ax * inv;
+      p.ny = ay * inv;
+    }
     p.nz = 0.0f;
-    if (g.beta > 0.0f) rough(rng, p.nx, p.ny, p.nz, g.beta);
+    if (g.beta > 0.0f) rough<Rng, M>(rng, p.nx, p.ny, p.nz, g.beta);
   } else {
     p.xc = p.px + R * (-p.nx);
     p.yc = p.py + R * (-p.ny);
@@ -787,7 +905,7 @@ reperturb_point:
   p.ux = odx; p.uy = ody; p.uz = odz;
   mx = -p.ux; my = -p.uy; mz = -p.uz;
   if (reperturb != 0) {
-    if (g.beta > 0.0f) rough(rng, p.nx, p.ny, p.nz, g.beta);
+    if (g.beta > 0.0f) rough<Rng, M>(rng, p.nx, p.ny, p.nz, g.beta);
     if (!inside) {
       p.xc = p.px + R * (-p.nx);
       p.yc = p.py + R * (-p.ny);
@@ -799,7 +917,8 @@ no_perturbation:
     float c1 = mx * p.nx + my * p.ny + mz * p.nz;  // cos(theta1)
     if (c1 > 1.0f || c1 < 0.0f) {
       if (!inside) {
-        if (column_normal(rng, p, g, mx, my, mz, normal_tries, false)) return kPrEnded;
+        if (column_normal<Rng, M>(rng, p, g, mx, my, mz, normal_tries, false))
+          return kPrEnded;
       }
       if (theta_tries < 100) {
         theta_tries++;
@@ -807,6 +926,41 @@ no_perturbation:
       }
       p.fate = kTheta1;
       return kPrEnded;
+    }
+    if constexpr (M::kFaithful) {
+      // K:1757-1871: theta1 = acosf; TIR against asin of the index ratio;
+      // theta1 < eps becomes 0.00042; theta2 = asinf((float)(ratio *
+      // sin(theta1))); Pr and Pt from the angles; then both normalised
+      t1 = fd2x_acosf(c1);
+      bool fresnel = false;
+      float snell = 0.0f;
+      if (inside) {
+        if (t1 > g.t1_tir_in) {
+          pr = 1.0f;
+          pt = 0.0f;
+        } else {
+          if (t1 < g.eps_lt) t1 = 0.00042f;
+          snell = g.ratio12;
+          fresnel = true;
+        }
+      } else if (!g.gap_fresnel) {
+        if (t1 > g.t1_tir_gap) {
+          pr = 1.0f;
+          pt = 0.0f;
+        }
+      } else {
+        snell = t1 < g.eps_lt ? g.ratio12 : g.ratio21;
+        if (t1 < g.eps_lt) t1 = 0.00042f;
+        fresnel = true;
+      }
+      if (fresnel) {
+        t2 = fd2x_asinf(fd2x_scaled_sin(snell, t1));
+        fd2x_fresnel(t1, t2, &pr, &pt);
+      }
+      const float tot = pr + pt;
+      pr = pr / tot;
+      pt = pt / tot;
+      goto fresnel_draw;
     }
     s1 = fd2_sqrt_unit(1.0f - c1 * c1);
     // theta1 < eps (c1 == 1) is replaced by 0.00042 rad where the reference
@@ -833,9 +987,10 @@ no_perturbation:
     }
   }
 
+fresnel_draw:
   if (rng.u() < pr) {
-    reflect(p, g);
-    if (angle_over_157(onx * p.ux + ony * p.uy + onz * p.uz)) {
+    reflect<M>(p, g);
+    if (angle_over<M>(onx * p.ux + ony * p.uy + onz * p.uz)) {
       reperturb++;
       if (reperturb < 4) goto reperturb_point;
       p.nx = onx; p.ny = ony; p.nz = onz;
@@ -852,7 +1007,10 @@ no_perturbation:
     if (!inside) goto gap_flight;
     return kPrAlive;
   }
-  refract(p, s1, s2, c2, g);
+  if constexpr (M::kFaithful)
+    refract_angles(p, t1, t2, g);
+  else
+    refract(p, s1, s2, c2, g);
   if (!inside) return kPrAlive;
   inside = false;
 gap_flight:
@@ -864,15 +1022,15 @@ gap_flight:
     st.c2 = c2;
     return kPrGap;
   }
-  if (transmit_gap(rng, p, g)) return kPrEnded;
+  if (transmit_gap<Rng, M>(rng, p, g)) return kPrEnded;
   goto prpt;
 }
 
 // The whole branch in one call.  true = ended.
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ bool ba_prpt(Rng& rng, Photon& p, const GeomDev& g) {
   PrptState st = prpt_initial();
-  return prpt_loop(rng, p, g, st, true, false) == kPrEnded;
+  return prpt_loop<Rng, M>(rng, p, g, st, true, false) == kPrEnded;
 }
 
 // One algo() pass of the reference (main/kernel_cuda_c_ver1_0.cu:553-655)
@@ -888,7 +1046,7 @@ __device__ __forceinline__ bool ba_prpt(Rng& rng, Photon& p, const GeomDev& g) {
 // not wait in a loop for lanes that need several.
 enum StepResult : int { kAlive = 0, kEnded = 1, kTrapped = 2, kNeedPrpt = 3 };
 
-template <class Rng, bool TRACE>
+template <class Rng, bool TRACE, class M = MathDevice>
 __device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
                                         StepSink* sink) {
   if (p.fate != kNewColumn) {
@@ -901,9 +1059,9 @@ __device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
       p.fate = kLost;
       return kEnded;
     }
-    if (isotropic<Rng, TRACE>(rng, p, g, sink)) return kEnded;
+    if (isotropic<Rng, TRACE, M>(rng, p, g, sink)) return kEnded;
     if (!resumed) {
-      const int d = detection(rng, p, g);
+      const int d = detection<Rng, M>(rng, p, g);
       if (d == kSensorDetect) return kEnded;
       if (d == kSensorReflect) {
         p.fate = kResume;
@@ -911,7 +1069,7 @@ __device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
       }
     }
     if (fabsf(p.uz) < g.eps_lt) return kTrapped;
-    const int b = ba_entry(rng, p, g);
+    const int b = ba_entry<Rng, M>(rng, p, g);
     if (b == kBaEnded) return kEnded;
     if (b == kBaPrpt) return kNeedPrpt;
     if (b == kBaAlive) return kAlive;
@@ -921,7 +1079,7 @@ __device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
   const int rejected = p.pix;
   p.pix = -1;
   p.fate = kNone;
-  if (column_normal_sample(rng, p, g, -p.ux, -p.uy, -p.uz, false)) {
+  if (column_normal_sample<Rng, M>(rng, p, g, -p.ux, -p.uy, -p.uz, false)) {
     p.xc = p.px + g.R * (-p.nx);
     p.yc = p.py + g.R * (-p.ny);
     return kAlive;
@@ -936,22 +1094,22 @@ __device__ __forceinline__ int hot_step(Rng& rng, Photon& p, const GeomDev& g,
 }
 
 // the whole pass, for kernels that do not defer the Fresnel branch
-template <class Rng, bool TRACE>
+template <class Rng, bool TRACE, class M = MathDevice>
 __device__ __forceinline__ int photon_step(Rng& rng, Photon& p,
                                            const GeomDev& g, StepSink* sink) {
-  const int r = hot_step<Rng, TRACE>(rng, p, g, sink);
+  const int r = hot_step<Rng, TRACE, M>(rng, p, g, sink);
   if (r != kNeedPrpt) return r;
-  return ba_prpt(rng, p, g) ? kEnded : kAlive;
+  return ba_prpt<Rng, M>(rng, p, g) ? kEnded : kAlive;
 }
 
 // trapped photon (|dcos.z| < eps): restart from the event with a new
 // direction; the column centre and rebound count are kept (C branch,
 // main/kernel_cuda_c_ver1_0.cu:609-644).  The normal drawn there is never
 // read before being overwritten, so only its two draws are consumed.
-template <class Rng>
+template <class Rng, class M = MathDevice>
 __device__ __forceinline__ void restart_trapped(Rng& rng, Photon& p,
                                                 const GeomDev& g, const fd2_event& ev) {
-  emit_dir(rng, p, g);
+  emit_dir<Rng, M>(rng, p, g);
   p.px = ev.x;
   p.py = ev.y;
   p.pz = ev.z;

@@ -97,6 +97,9 @@ void derive(const fd2_params& p, GeomDev& g) {
   g.cos_tir_gap = g.ratio12 <= 1.0f
                       ? float_ge(std::sqrt(1.0 - (double)g.ratio12 * (double)g.ratio12))
                       : NAN;
+  // theta1 > asin(r) (a double) <=> theta1 > the largest float <= asin(r)
+  g.t1_tir_in = float_le(std::asin((double)g.ratio21));
+  g.t1_tir_gap = g.ratio12 <= 1.0f ? float_le(std::asin((double)g.ratio12)) : NAN;
   g.pixel = p.detpixel;
   g.inv_pixel = (p.detpixel & (p.detpixel - 1)) == 0 ? 1.0f / (float)p.detpixel : 0.0f;
   g.xdim = prf_side(p.detlboundx, p.detuboundx, p.detpixel);
@@ -203,6 +206,7 @@ Launch make_launch(const fd2_ctx* c, const QueueBufs& b, const fd2_event* d_ev, 
   q.n_geoms = c->n_geoms;
   q.chunk_log2 = ilog2(c->tune.chunk);
   q.n_blocks = (int)((n + kScanTile - 1) / kScanTile);
+  q.max_chunks = (uint64_t)c->tune.max_chunks;
 
   TransportArgs& a = L.a;
   a.events = d_ev;
@@ -219,7 +223,7 @@ Launch make_launch(const fd2_ctx* c, const QueueBufs& b, const fd2_event* d_ev, 
   a.error_flag = c->d_error;
   a.n_events = n;
   a.seed = c->seed;
-  a.event_base = (uint32_t)c->next_event;
+  a.event_base = (uint64_t)c->next_event;
   a.n_geoms = c->n_geoms;
   a.chunk_log2 = q.chunk_log2;
   a.park_lanes = c->tune.park_lanes;
@@ -244,24 +248,50 @@ hipError_t issue(const Launch& L, hipStream_t s) {
   return launch_transport(L.a, L.multi, L.record, L.radial, L.blocks, s);
 }
 
-// queue build + transport on events already in device memory
+// queue build + transport on events already in device memory.  With the
+// events' host copy h_ev, the events are split into consecutive launches of
+// at most tune.max_chunks work items each (each launch numbers its events
+// on from the last, so the photons are those of one launch); without it the
+// queue build rejects a launch above the limit (error bit 4).
 int enqueue(fd2_ctx* c, const fd2_event* d_ev, int64_t n, bool record,
-            fd2_photon* d_records, uint32_t* d_radial = nullptr) {
+            fd2_photon* d_records, uint32_t* d_radial = nullptr,
+            const fd2_event* h_ev = nullptr) {
   if (n <= 0) return FD2_OK;
   int rc = grow(c, (size_t)n);
   if (rc) return rc;
   const QueueBufs b{c->d_chunk_off, c->d_photon_off, c->d_block_chunks, c->d_block_photons};
-  const Launch L = make_launch(c, b, d_ev, n, record ? d_records : nullptr, d_radial);
+  const uint64_t chunk = (uint64_t)c->tune.chunk, limit = (uint64_t)c->tune.max_chunks;
   FD2_HIP(hipEventRecord(c->ev_start, c->stream));
-  FD2_HIP(issue(L, c->stream));
+  uint64_t photons_before = 0;
+  for (int64_t i0 = 0; i0 < n;) {
+    int64_t i1 = n;
+    uint64_t photons = 0;
+    if (h_ev) {
+      uint64_t items = 0;
+      for (i1 = i0; i1 < n; ++i1) {
+        const uint64_t k = ((uint64_t)h_ev[i1].N + chunk - 1) / chunk;
+        if (items + k > limit && i1 > i0) break;
+        items += k;
+        photons += (uint64_t)h_ev[i1].N;
+      }
+    }
+    const Launch L =
+        make_launch(c, b, d_ev + i0, i1 - i0,
+                    record ? d_records + photons_before : nullptr,
+                    d_radial ? d_radial + (size_t)i0 * FD2_RADIAL_BINS : nullptr);
+    FD2_HIP(issue(L, c->stream));
+    c->next_event += i1 - i0;
+    photons_before += photons;
+    i0 = i1;
+  }
   FD2_HIP(hipEventRecord(c->ev_stop, c->stream));
   c->timed = true;
-  c->next_event += n;
   return FD2_OK;
 }
 
 // bit 0: an event failed validation in the queue build; bit 1: a wave of the
-// persistent kernel hit its loop limit (results incomplete)
+// persistent kernel hit its loop limit (results incomplete); bit 2: a launch
+// had more work items than tune.max_chunks (nothing of it ran)
 int check_error_flag(fd2_ctx* c) {
   uint32_t flag = 0;
   FD2_HIP(hipMemcpyAsync(&flag, c->d_error, sizeof flag, hipMemcpyDeviceToHost, c->stream));
@@ -294,7 +324,7 @@ int submit_radial_async(fd2_ctx* c, const fd2_event* events, int64_t n, uint32_t
   FD2_HIP(hipMemcpyAsync(c->d_events, c->h_stage, (size_t)n * sizeof(fd2_event),
                          hipMemcpyHostToDevice, c->stream));
   FD2_HIP(launch_fill_u32(d_radial, (size_t)n * FD2_RADIAL_BINS, c->stream));
-  return enqueue(c, c->d_events, n, false, nullptr, d_radial);
+  return enqueue(c, c->d_events, n, false, nullptr, d_radial, c->h_stage);
 }
 
 }  // namespace fd2
@@ -333,6 +363,7 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
   c->tune.park_lanes = 32;
   c->tune.refill_lanes = 4;
   c->tune.wall_queue = 0;
+  c->tune.max_chunks = FD2_MAX_CHUNKS;
   int rc = FD2_OK;
   do {
     if (hipSetDevice(device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
@@ -443,6 +474,10 @@ int fd2_set_tuning(fd2_ctx* c, const fd2_tuning* t) {
   }
   if (t->wall_queue < -1 || t->wall_queue > 1) return FD2_E_INVAL;
   n.wall_queue = t->wall_queue;
+  if (t->max_chunks) {
+    if (t->max_chunks < 1 || t->max_chunks > FD2_MAX_CHUNKS) return FD2_E_INVAL;
+    n.max_chunks = t->max_chunks;
+  }
   c->tune = n;
   return FD2_OK;
 }
@@ -468,6 +503,12 @@ int fd2_reset(fd2_ctx* c) {
   return FD2_OK;
 }
 
+int fd2_set_next_event(fd2_ctx* c, int64_t first) {
+  if (!c || first < 0) return FD2_E_INVAL;
+  c->next_event = first;
+  return FD2_OK;
+}
+
 int fd2_submit(fd2_ctx* c, const fd2_event* events, int64_t n) {
   if (!c || n < 0 || (n > 0 && !events)) return FD2_E_INVAL;
   if (n == 0) return FD2_OK;
@@ -483,7 +524,7 @@ int fd2_submit(fd2_ctx* c, const fd2_event* events, int64_t n) {
   std::memcpy(c->h_stage, events, (size_t)n * sizeof(fd2_event));
   FD2_HIP(hipMemcpyAsync(c->d_events, c->h_stage, (size_t)n * sizeof(fd2_event),
                          hipMemcpyHostToDevice, c->stream));
-  return enqueue(c, c->d_events, n, false, nullptr);
+  return enqueue(c, c->d_events, n, false, nullptr, nullptr, c->h_stage);
 }
 
 int fd2_submit_device(fd2_ctx* c, const fd2_event* d_events, int64_t n) {
@@ -515,7 +556,7 @@ int fd2_submit_record(fd2_ctx* c, const fd2_event* events, int64_t n,
                          hipMemcpyHostToDevice, c->stream));
   fd2_photon* d_rec = nullptr;
   if (total > 0) FD2_HIP(hipMalloc(&d_rec, (size_t)total * sizeof(fd2_photon)));
-  rc = enqueue(c, c->d_events, n, true, d_rec);
+  rc = enqueue(c, c->d_events, n, true, d_rec, nullptr, events);
   if (rc == FD2_OK && total > 0)
     rc = hipMemcpyAsync(records, d_rec, (size_t)total * sizeof(fd2_photon),
                         hipMemcpyDeviceToHost, c->stream) == hipSuccess ? FD2_OK : FD2_E_DEVICE;
@@ -545,10 +586,13 @@ int fd2_submit_radial(fd2_ctx* c, const fd2_event* events, int64_t n,
   return check_error_flag(c);
 }
 
-int fd2_trace_ranecu(fd2_ctx* c, const fd2_event* events, int64_t n,
-                     int64_t event_base, int32_t seed_input, fd2_photon* records) {
+static int trace_ranecu(fd2_ctx* c, const fd2_event* events, int64_t n,
+                        int64_t event_base, int32_t seed_input, fd2_photon* records,
+                        bool faithful) {
   if (!c || n < 0 || (n > 0 && !events)) return FD2_E_INVAL;
   if (n == 0) return FD2_OK;
+  // the reference's init_PRNG takes the event index as an int
+  if (event_base < 0 || event_base > (int64_t)INT32_MAX - n + 1) return FD2_E_INVAL;
   int rc = validate_host(c, events, n);
   if (rc) return rc;
   FD2_HIP(hipSetDevice(c->device));
@@ -570,7 +614,8 @@ int fd2_trace_ranecu(fd2_ctx* c, const fd2_event* events, int64_t n,
     rc = FD2_E_DEVICE;
   if (!rc && records && acc > 0 && hipMalloc(&d_rec, acc * sizeof(fd2_photon)) != hipSuccess) rc = FD2_E_NOMEM;
   if (!rc && launch_trace_ranecu(c->d_events, n, event_base, seed_input, c->d_geoms, c->n_geoms,
-                                 c->d_counters, c->d_photon_off, d_rec, c->stream) != hipSuccess)
+                                 c->d_counters, c->d_photon_off, d_rec, faithful,
+                                 c->stream) != hipSuccess)
     rc = FD2_E_DEVICE;
   if (!rc && d_rec &&
       hipMemcpyAsync(records, d_rec, acc * sizeof(fd2_photon), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
@@ -579,6 +624,16 @@ int fd2_trace_ranecu(fd2_ctx* c, const fd2_event* events, int64_t n,
   if (d_rec) (void)hipFree(d_rec);
   std::free(h_off);
   return rc;
+}
+
+int fd2_trace_ranecu(fd2_ctx* c, const fd2_event* events, int64_t n,
+                     int64_t event_base, int32_t seed_input, fd2_photon* records) {
+  return trace_ranecu(c, events, n, event_base, seed_input, records, false);
+}
+
+int fd2_trace_ranecu_faithful(fd2_ctx* c, const fd2_event* events, int64_t n,
+                              int64_t event_base, int32_t seed_input, fd2_photon* records) {
+  return trace_ranecu(c, events, n, event_base, seed_input, records, true);
 }
 
 int fd2_debug_steps(fd2_ctx* c, const fd2_event* events, int64_t n, int32_t max_photons,
@@ -603,7 +658,7 @@ int fd2_debug_steps(fd2_ctx* c, const fd2_event* events, int64_t n, int32_t max_
   int32_t cnt = 0;
   if (hipMemcpyAsync(c->d_events, events, sizeof(fd2_event), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       launch_fill_u32((uint32_t*)d_count, 1, c->stream) != hipSuccess ||
-      launch_debug_steps(c->d_events, (uint32_t)c->next_event, c->seed, c->d_geoms, max_photons,
+      launch_debug_steps(c->d_events, (uint64_t)c->next_event, c->seed, c->d_geoms, max_photons,
                          d_steps, d_count, max_steps, c->stream) != hipSuccess ||
       hipMemcpyAsync(&cnt, d_count, sizeof cnt, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
@@ -759,7 +814,7 @@ struct fd2_graph {
   unsigned long long* d_photon_off = nullptr;
   uint32_t* d_block_chunks = nullptr;
   unsigned long long* d_block_photons = nullptr;
-  uint32_t* d_base = nullptr;
+  unsigned long long* d_base = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -796,7 +851,7 @@ int fd2_graph_create(fd2_ctx* c, const fd2_event* d_events, int64_t n, fd2_graph
       hipMalloc(&g->d_photon_off, (size_t)(n + 1) * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&g->d_block_chunks, nb * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&g->d_block_photons, nb * sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&g->d_base, sizeof(uint32_t)) != hipSuccess)
+      hipMalloc(&g->d_base, sizeof(unsigned long long)) != hipSuccess)
     rc = FD2_E_NOMEM;
   if (!rc) {
     const QueueBufs b{g->d_chunk_off, g->d_photon_off, g->d_block_chunks, g->d_block_photons};
@@ -824,10 +879,10 @@ int fd2_graph_create(fd2_ctx* c, const fd2_event* d_events, int64_t n, fd2_graph
 int fd2_graph_launch(fd2_graph* g) {
   if (!g || !g->exec) return FD2_E_INVAL;
   fd2_ctx* c = g->c;
-  if (c->next_event + g->n > (int64_t)UINT32_MAX) return FD2_E_STATE;
+  if (g->n > INT64_MAX - c->next_event) return FD2_E_STATE;
   FD2_HIP(hipSetDevice(c->device));
   FD2_HIP(hipEventRecord(c->ev_start, c->stream));
-  FD2_HIP(launch_set_u32(g->d_base, (uint32_t)c->next_event, c->stream));
+  FD2_HIP(launch_set_u64(g->d_base, (unsigned long long)c->next_event, c->stream));
   FD2_HIP(hipGraphLaunch(g->exec, c->stream));
   FD2_HIP(hipEventRecord(c->ev_stop, c->stream));
   c->timed = true;

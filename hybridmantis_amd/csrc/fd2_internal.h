@@ -19,12 +19,12 @@ struct TransportArgs {
   uint32_t* cursor;                    // work-queue cursor (zeroed by the scan)
   fd2_photon* records;                 // per-photon records (recording only)
   uint32_t* radial;                    // n_events x FD2_RADIAL_BINS (radial mode)
-  const uint32_t* event_base_dev;      // if set, read the number of events[0] here
+  const unsigned long long* event_base_dev;  // if set, read the number of events[0] here
   uint32_t* wave_queue;                // per-wave photon queues (null: park in place)
   uint32_t* error_flag;                // bit 1: a wave's loop limit was reached
   int64_t n_events;
   uint32_t seed;
-  uint32_t event_base;                 // global number of events[0]
+  uint64_t event_base;                 // global number of events[0]
   int32_t n_geoms;
   int32_t chunk_log2;
   int32_t park_lanes;                  // lanes waiting before a Fresnel phase
@@ -68,6 +68,7 @@ struct QueueArgs {
   int32_t n_geoms;
   int32_t chunk_log2;
   int32_t n_blocks;
+  uint64_t max_chunks;         // more work items than this: error bit 4, nothing runs
 };
 
 constexpr int kScanThreads = 1024;
@@ -84,8 +85,8 @@ hipError_t launch_trace_ranecu(const fd2_event* events, int64_t n,
                                const GeomDev* geoms, int32_t n_geoms,
                                unsigned long long* counters,
                                const unsigned long long* photon_off,
-                               fd2_photon* records, hipStream_t s);
-hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
+                               fd2_photon* records, bool faithful, hipStream_t s);
+hipError_t launch_debug_steps(const fd2_event* events, uint64_t event_base,
                               uint32_t seed, const GeomDev* geoms,
                               int32_t n_photons, float* steps,
                               int32_t* n_steps, int32_t max_steps,
@@ -94,6 +95,7 @@ hipError_t launch_debug_math(int fn, const float* x, float* y, int64_t n, hipStr
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+hipError_t launch_set_u64(unsigned long long* p, unsigned long long v, hipStream_t s);
 int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem, bool queue);
 
 }  // namespace fd2

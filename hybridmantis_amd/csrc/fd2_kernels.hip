@@ -90,11 +90,14 @@ __global__ void __launch_bounds__(kScanThreads) k_queue_totals(QueueArgs q) {
   }
 }
 
-// pass 2: scan of the block totals (one block), grand totals, cursor reset
+// pass 2: scan of the block totals (one block), grand totals, cursor reset.
+// The work-item total is summed in 64 bits: a launch of more than
+// q.max_chunks items (which the 32-bit offsets and cursor cannot number)
+// sets error bit 4 and gets a total of 0, so the transport takes nothing.
 __global__ void __launch_bounds__(kScanThreads) k_queue_blocks(QueueArgs q) {
   __shared__ uint32_t s_c[kScanThreads / 64 + 1];
   __shared__ unsigned long long s_p[kScanThreads / 64 + 1];
-  uint32_t carry_c = 0;
+  uint64_t carry_c = 0;
   unsigned long long carry_p = 0;
   for (int b0 = 0; b0 < q.n_blocks; b0 += kScanThreads) {
     const int b = b0 + threadIdx.x;
@@ -105,14 +108,16 @@ __global__ void __launch_bounds__(kScanThreads) k_queue_blocks(QueueArgs q) {
     const uint32_t ec = block_excl_scan<uint32_t>(c, s_c, tc);
     const unsigned long long ep = block_excl_scan<unsigned long long>(p, s_p, tp);
     if (b < q.n_blocks) {
-      q.block_chunks[b] = carry_c + ec;
+      q.block_chunks[b] = (uint32_t)(carry_c + ec);
       q.block_photons[b] = carry_p + ep;
     }
     carry_c += tc;
     carry_p += tp;
   }
   if (threadIdx.x == 0) {
-    q.chunk_off[q.n_events] = carry_c;
+    const bool over = carry_c > q.max_chunks;
+    if (over) atomicOr(q.error_flag, 4u);
+    q.chunk_off[q.n_events] = over ? 0u : (uint32_t)carry_c;
     q.photon_off[q.n_events] = carry_p;
     *q.cursor = 0u;
   }
@@ -450,7 +455,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
   uint32_t* run = s_run[threadIdx.x];
   run[0] = 0u;
   run[1] = 0u;
-  const uint32_t event_base = a.event_base_dev ? *a.event_base_dev : a.event_base;
+  const uint64_t event_base = a.event_base_dev ? *a.event_base_dev : a.event_base;
 
   const int lane = threadIdx.x & 63;
   const uint32_t chunk = 1u << a.chunk_log2;
@@ -539,7 +544,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
           const uint32_t k = (uint32_t)(start_base + lane);
           CounterRng r;
           Photon q;
-          r.seed(a.seed, event_base + (uint32_t)pool_ev, k);
+          r.seed(a.seed, event_base + (uint64_t)pool_ev, k);
           counter_initial_dir(r, q, k, geoms[MULTI ? events[pool_ev].geom : 0u]);
           start[0][lane] = __float_as_uint(q.ux);
           start[1][lane] = __float_as_uint(q.uy);
@@ -573,7 +578,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
         parked = false;
         alive = true;
       }
-      pool_next += min(__popcll(need), avail);
+      pool_next += min((int)__popcll(need), avail);
       need = __ballot(!alive);
     }
     // with no lane alive the work queue and the backlog are empty
@@ -804,6 +809,7 @@ int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t sme
 
 // ------------------------------------------------ RANECU trajectory mode
 
+template <class M>
 __global__ void __launch_bounds__(64) k_trace_ranecu(
     const fd2_event* events, int64_t n, int64_t event_base, int32_t seed_input,
     const GeomDev* geoms, unsigned long long* counters,
@@ -818,7 +824,7 @@ __global__ void __launch_bounds__(64) k_trace_ranecu(
   StepSink nosink{nullptr, nullptr, 0, 0, false};
   Photon p;
   start_photon(p, ev);
-  if (ev.N > 0) emit_first_dir(rng, p, g);  // driver, normal stays 0
+  if (ev.N > 0) emit_first_dir<RanecuRng, M>(rng, p, g);  // driver, normal stays 0
   for (int32_t k = 0; k < ev.N; ++k) {
     // direction and normal carry over from the previous photon's end
     const float ux = p.ux, uy = p.uy, uz = p.uz, nx = p.nx, ny = p.ny, nz = p.nz;
@@ -827,11 +833,11 @@ __global__ void __launch_bounds__(64) k_trace_ranecu(
     p.nx = nx; p.ny = ny; p.nz = nz;
     cnt[FD2_C_GENERATED]++;
     for (;;) {
-      const int r = photon_step<RanecuRng, false>(rng, p, g, &nosink);
+      const int r = photon_step<RanecuRng, false, M>(rng, p, g, &nosink);
       if (r == kEnded) break;
       if (r == kTrapped) {
         cnt[FD2_C_REGEN]++;
-        restart_trapped(rng, p, g, ev);
+        restart_trapped<RanecuRng, M>(rng, p, g, ev);
       }
     }
     cnt[fate_counter(p.fate)]++;
@@ -847,7 +853,7 @@ __global__ void __launch_bounds__(64) k_trace_ranecu(
       records[photon_off[i] + k] = rec;
     }
     // next photon's direction and normal draws (main/kernel_cuda_c_ver1_0.cu:578-602)
-    emit_dir(rng, p, g);
+    emit_dir<RanecuRng, M>(rng, p, g);
     if (g.beta > 0.0f) {
       (void)rng.u();
       (void)rng.u();
@@ -863,19 +869,23 @@ hipError_t launch_trace_ranecu(const fd2_event* events, int64_t n,
                                const GeomDev* geoms, int32_t n_geoms,
                                unsigned long long* counters,
                                const unsigned long long* photon_off,
-                               fd2_photon* records, hipStream_t s) {
+                               fd2_photon* records, bool faithful, hipStream_t s) {
   (void)n_geoms;
   if (n <= 0) return hipSuccess;
   const int64_t nb = (n + 63) / 64;
-  hipLaunchKernelGGL(k_trace_ranecu, dim3((unsigned)nb), dim3(64), 0, s, events, n,
-                     event_base, seed_input, geoms, counters, photon_off, records);
+  if (faithful)
+    hipLaunchKernelGGL(k_trace_ranecu<MathFaithful>, dim3((unsigned)nb), dim3(64), 0, s, events,
+                       n, event_base, seed_input, geoms, counters, photon_off, records);
+  else
+    hipLaunchKernelGGL(k_trace_ranecu<MathDevice>, dim3((unsigned)nb), dim3(64), 0, s, events,
+                       n, event_base, seed_input, geoms, counters, photon_off, records);
   return hipGetLastError();
 }
 
 // ----------------------------------------------------- step trace (debug)
 
 __global__ void __launch_bounds__(64) k_debug_steps(
-    const fd2_event* events, uint32_t event_base, uint32_t seed,
+    const fd2_event* events, uint64_t event_base, uint32_t seed,
     const GeomDev* geoms, int32_t n_photons, float* steps, int32_t* n_steps,
     int32_t max_steps) {
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -895,7 +905,7 @@ __global__ void __launch_bounds__(64) k_debug_steps(
   }
 }
 
-hipError_t launch_debug_steps(const fd2_event* events, uint32_t event_base,
+hipError_t launch_debug_steps(const fd2_event* events, uint64_t event_base,
                               uint32_t seed, const GeomDev* geoms,
                               int32_t n_photons, float* steps,
                               int32_t* n_steps, int32_t max_steps,
@@ -927,6 +937,11 @@ __global__ void k_debug_math(int fn, const float* x, float* y, int64_t n) {
     case FD2_MATH_RSQRT: r = fd2_rsqrt(v); break;
     case FD2_MATH_SQRT_UNIT: r = fd2_sqrt_unit(v); break;
     case FD2_MATH_ANGLE_OVER_157: r = angle_over_157(v) ? 1.0f : 0.0f; break;
+    case FD2_MATH_ACOSF_LIBC: r = fd2x_acosf(v); break;
+    case FD2_MATH_ASINF_LIBC: r = fd2x_asinf(v); break;
+    case FD2_MATH_FF_SIN: r = fd2x_sin(v).hi; break;
+    case FD2_MATH_FF_COS: r = fd2x_cos(v).hi; break;
+    case FD2_MATH_FF_LOG: r = fd2x_log(v).hi; break;
     default: r = 0.0f; break;
   }
   y[i] = r;
@@ -971,6 +986,15 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v) {
 
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
+  if (threadIdx.x == 0) p[0] = v;
+}
+
+hipError_t launch_set_u64(unsigned long long* p, unsigned long long v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, p, v);
   return hipGetLastError();
 }
 

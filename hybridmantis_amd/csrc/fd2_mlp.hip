@@ -716,8 +716,8 @@ __global__ void k_surr_features(const fd2_event* ev, int n, const fd2::GeomDev* 
 // Network events gathered from the whole submitted array by index; their
 // random-stream keys are the global event numbers base + index.
 __global__ void k_surr_gather(const fd2_event* __restrict__ all, const uint32_t* __restrict__ idx,
-                              int64_t n, uint32_t base, fd2_event* __restrict__ out,
-                              uint32_t* __restrict__ ids) {
+                              int64_t n, uint64_t base, fd2_event* __restrict__ out,
+                              uint64_t* __restrict__ ids) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t k = idx[i];
@@ -734,7 +734,7 @@ __device__ __forceinline__ float radial_edge(int j) {
 
 struct SampleArgs {
   const fd2_event* events;
-  const uint32_t* ids;       // global event numbers (random-stream keys)
+  const uint64_t* ids;       // global event numbers (random-stream keys)
   const float* logits;       // n x kOutPad
   const fd2::GeomDev* geoms;
   unsigned long long* counters;
@@ -765,7 +765,7 @@ constexpr int kSpreadCopies = 8;
 template <int THREADS>
 __global__ void __launch_bounds__(THREADS)
 k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
-              const uint32_t* __restrict__ ids, const float* __restrict__ logits,
+              const uint64_t* __restrict__ ids, const float* __restrict__ logits,
               const fd2::GeomDev* __restrict__ geoms) {
   extern __shared__ uint32_t s_tile[];
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * 3];
@@ -945,7 +945,9 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
         return;
       }
       const fd2::GeomDev& g = c->hgeom[e.geom];
-      const uint32_t gid = (uint32_t)(base + i);
+      // audit hash of the 64-bit event number (its low word below 2^32)
+      const uint64_t gid64 = (uint64_t)(base + i);
+      const uint32_t gid = (uint32_t)gid64 ^ ((uint32_t)(gid64 >> 32) * 0x9E3779B9u);
       const bool near = e.z > g.half_h - cfg.surface_band || e.z < -g.half_h + cfg.surface_band;
       const float ox = e.x - 0.5f * g.xdet, oy = e.y - 0.5f * g.ydet;
       const bool off_centre = ox * ox + oy * oy > cfg.center_band * cfg.center_band;
@@ -1034,7 +1036,7 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
   fd2_event* d_all = nullptr;
   fd2_event* d_ev = nullptr;
   uint32_t* d_idx = nullptr;
-  uint32_t* d_ids = nullptr;
+  uint64_t* d_ids = nullptr;
   uint32_t* d_rad = nullptr;
   float* d_logits = nullptr;
   uint32_t* d_spread = nullptr;
@@ -1119,7 +1121,7 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
     if (hipMalloc(&d_all, (size_t)n * sizeof(fd2_event)) != hipSuccess ||
         hipMalloc(&d_ev, (size_t)ns * sizeof(fd2_event)) != hipSuccess ||
         hipMalloc(&d_idx, (size_t)ns * 4) != hipSuccess ||
-        hipMalloc(&d_ids, (size_t)ns * 4) != hipSuccess ||
+        hipMalloc(&d_ids, (size_t)ns * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc(&d_logits, (size_t)ns * kOutPad * sizeof(float)) != hipSuccess)
       return release(FD2_E_NOMEM);
     const size_t npix0 = (size_t)c->hgeom[0].xdim * c->hgeom[0].ydim;
@@ -1164,7 +1166,7 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
     if (fd2::launch_fill_u32(d_spread, npix0 * fd2mlp::kSpreadCopies, p->stream) != hipSuccess)
       return release(FD2_E_DEVICE);
     hipLaunchKernelGGL(fd2mlp::k_surr_gather, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
-                       p->stream, d_all, d_idx, ns, (uint32_t)base, d_ev, d_ids);
+                       p->stream, d_all, d_idx, ns, (uint64_t)base, d_ev, d_ids);
     for (int64_t s0 = 0; s0 < ns && rc == FD2_OK; s0 += chunk) {
       const int b = (int)(ns - s0 < chunk ? ns - s0 : chunk);
       hipLaunchKernelGGL(fd2mlp::k_surr_features, dim3((b * kInPad + 255) / 256), dim3(256), 0,

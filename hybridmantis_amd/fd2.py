@@ -46,10 +46,15 @@ class Params(C.Structure):
                                  "mynumbins")]
 
 
+def _tuning_dict(t) -> dict:
+    return {name: getattr(t, name) for name, _ in Tuning._fields_}
+
+
 class Tuning(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("waves_per_cu", C.c_int32),
                 ("lds_tile", C.c_int32), ("park_lanes", C.c_int32),
-                ("refill_lanes", C.c_int32), ("wall_queue", C.c_int32)]
+                ("refill_lanes", C.c_int32), ("wall_queue", C.c_int32),
+                ("max_chunks", C.c_int64)]
 
 
 # dict keys (events.BASE_PARAMS) -> fd2_params members
@@ -95,6 +100,8 @@ def lib() -> C.CDLL:
         "fd2_sync": ([P], C.c_int),
         "fd2_submit_record": ([P, P, C.c_int64, P], C.c_int),
         "fd2_trace_ranecu": ([P, P, C.c_int64, C.c_int64, C.c_int32, P], C.c_int),
+        "fd2_trace_ranecu_faithful": ([P, P, C.c_int64, C.c_int64, C.c_int32, P], C.c_int),
+        "fd2_set_next_event": ([P, C.c_int64], C.c_int),
         "fd2_debug_steps": ([P, P, C.c_int64, C.c_int32, P, C.c_int32,
                              C.POINTER(C.c_int32)], C.c_int),
         "fd2_debug_math": ([C.c_int, P, P, C.c_int64, C.c_int], C.c_int),
@@ -131,7 +138,8 @@ def _check(rc: int, what: str) -> None:
 
 # fd2_debug_math function numbers (FD2_MATH_* in fd2.h)
 MATH_FN = {"sin": 0, "cos": 1, "log": 2, "asin": 3, "acos": 4, "sin_turns": 5,
-           "cos_turns": 6, "rsqrt": 7, "sqrt_unit": 8, "angle_over_157": 9}
+           "cos_turns": 6, "rsqrt": 7, "sqrt_unit": 8, "angle_over_157": 9,
+           "acosf_libc": 10, "asinf_libc": 11, "ff_sin": 12, "ff_cos": 13, "ff_log": 14}
 
 
 def debug_math(fn: str, x, device: int = 0) -> np.ndarray:
@@ -250,16 +258,20 @@ class Transport:
 
     # -- configuration
     def set_tuning(self, chunk: int = 0, waves_per_cu: int = 0, lds_tile: int = 0,
-                   park_lanes: int = 0, refill_lanes: int = 0, wall_queue: int = 0):
-        t = Tuning(chunk, waves_per_cu, lds_tile, park_lanes, refill_lanes, wall_queue)
+                   park_lanes: int = 0, refill_lanes: int = 0, wall_queue: int = 0,
+                   max_chunks: int = 0):
+        t = Tuning(chunk, waves_per_cu, lds_tile, park_lanes, refill_lanes, wall_queue,
+                   max_chunks)
         _check(lib().fd2_set_tuning(self._ctx, C.byref(t)), "set_tuning")
 
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().fd2_get_tuning(self._ctx, C.byref(t)), "get_tuning")
-        return dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu, lds_tile=t.lds_tile,
-                    park_lanes=t.park_lanes, refill_lanes=t.refill_lanes,
-                    wall_queue=t.wall_queue)
+        return _tuning_dict(t)
+
+    def set_next_event(self, first: int) -> None:
+        """Number the next submitted event `first` (fd2_set_next_event)."""
+        _check(lib().fd2_set_next_event(self._ctx, first), "set_next_event")
 
     def set_seed(self, seed: int) -> None:
         _check(lib().fd2_set_seed(self._ctx, seed & 0xFFFFFFFF), "set_seed")
@@ -283,13 +295,18 @@ class Transport:
         return rec
 
     def trace_ranecu(self, events, event_base: int = 0,
-                     seed_input: int = 271828182, record: bool = True):
+                     seed_input: int = 271828182, record: bool = True,
+                     faithful: bool = False):
+        """One lane per event on the reference's per-event RANECU stream;
+        faithful=True runs it in the reference C path's own arithmetic
+        (fd2_trace_ranecu_faithful), else in the production FP32 arithmetic."""
         ev = as_events(events)
         rec = np.zeros(int(ev["N"].sum()), PHOTON_DTYPE) if record else None
-        _check(lib().fd2_trace_ranecu(self._ctx, ev.ctypes.data, len(ev), event_base,
-                                      seed_input,
-                                      rec.ctypes.data if rec is not None else None),
-               "fd2_trace_ranecu")
+        name = "fd2_trace_ranecu_faithful" if faithful else "fd2_trace_ranecu"
+        _check(getattr(lib(), name)(self._ctx, ev.ctypes.data, len(ev), event_base,
+                                    seed_input,
+                                    rec.ctypes.data if rec is not None else None),
+               name)
         return rec
 
     def debug_steps(self, events, max_photons: int, max_steps: int) -> np.ndarray:
@@ -326,10 +343,8 @@ class Transport:
         nres = C.c_int32(0)
         _check(lib().fd2_autotune(self._ctx, buf.ptr, n, repeats, C.byref(best), res,
                                   max_results, C.byref(nres)), "fd2_autotune")
-        as_dict = lambda t: dict(chunk=t.chunk, waves_per_cu=t.waves_per_cu,  # noqa: E731
-                                 lds_tile=t.lds_tile, park_lanes=t.park_lanes,
-                                 refill_lanes=t.refill_lanes, wall_queue=t.wall_queue)
-        return as_dict(best), [(as_dict(r.tuning), float(r.ms)) for r in res[:nres.value]]
+        return _tuning_dict(best), [(_tuning_dict(r.tuning), float(r.ms))
+                                    for r in res[:nres.value]]
 
     # -- tallies
     def prf_dims(self, geom: int = 0):

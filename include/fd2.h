@@ -119,7 +119,15 @@ typedef struct fd2_tuning {
                              * 1 = move to a per-wave queue (the branch runs
                              * with 64 lanes), -1 = park in their lanes
                              * (park_lanes applies); 0 = default (1)        */
+  int64_t max_chunks;       /* work items (chunks) per launch, at most
+                             * FD2_MAX_CHUNKS (the 32-bit queue cursor's
+                             * range); 0 = default (FD2_MAX_CHUNKS).  A
+                             * submit from host memory above it runs as
+                             * several launches (same photons); a device
+                             * submit or graph above it fails with
+                             * FD2_E_INVAL at fd2_sync and tallies nothing  */
 } fd2_tuning;
+#define FD2_MAX_CHUNKS 4294901760LL /* 2^32 - 2^16 */
 
 typedef struct fd2_ctx fd2_ctx;
 
@@ -165,6 +173,17 @@ int fd2_trace_ranecu(fd2_ctx *ctx, const fd2_event *events, int64_t n,
                      int64_t event_base, int32_t seed_input,
                      fd2_photon *records);
 
+/* The same trajectory kernel in the reference C path's own arithmetic
+ * (float state, each double <math.h> expression evaluated with pairs of
+ * floats and rounded to float where the reference rounds it, the C
+ * library's acosf/asinf restated; no FP64 on the GPU), so that its photons
+ * follow the reference's own trajectories on the same streams
+ * (main/kernel_cuda_c_ver1_0.cu C branch, main/hybridMANTIS_c_ver1_0.c:
+ * 237-299).  A test path: slow, one lane per event. */
+int fd2_trace_ranecu_faithful(fd2_ctx *ctx, const fd2_event *events, int64_t n,
+                              int64_t event_base, int32_t seed_input,
+                              fd2_photon *records);
+
 /* Debug: the in-column steps (dist_to_surface) of the first `max_photons`
  * photons of events[0], FP32 state before and after each step; returns the
  * number of steps written (<= max_steps) in *n_steps.  Synchronous. */
@@ -185,12 +204,17 @@ int fd2_debug_steps(fd2_ctx *ctx, const fd2_event *events, int64_t n,
  * on GPU `device` (synchronous; no context).  Compared bit for bit with the
  * same functions on the CPU by the tests.  FD2_MATH_SQRT_UNIT is defined for
  * x = +-0, x >= 2^-96, +inf and NaN; FD2_MATH_ANGLE_OVER_157 gives 1 when
- * acos x < 0 or > 1.57 (the new-column and reflection angle tests), else 0. */
+ * acos x < 0 or > 1.57 (the new-column and reflection angle tests), else 0.
+ * The faithful arithmetic's pieces: FD2_MATH_ACOSF_LIBC / ASINF_LIBC the C
+ * library's float arc functions as restated; FF_SIN / FF_COS / FF_LOG the
+ * float-pair sin, cos (|x| <= 8) and log (x > 0) rounded to float. */
 enum {
   FD2_MATH_SIN = 0, FD2_MATH_COS = 1, FD2_MATH_LOG = 2, FD2_MATH_ASIN = 3,
   FD2_MATH_ACOS = 4, FD2_MATH_SIN_TURNS = 5, FD2_MATH_COS_TURNS = 6,
   FD2_MATH_RSQRT = 7, FD2_MATH_SQRT_UNIT = 8, FD2_MATH_ANGLE_OVER_157 = 9,
-  FD2_MATH_NFN = 10
+  FD2_MATH_ACOSF_LIBC = 10, FD2_MATH_ASINF_LIBC = 11, FD2_MATH_FF_SIN = 12,
+  FD2_MATH_FF_COS = 13, FD2_MATH_FF_LOG = 14,
+  FD2_MATH_NFN = 15
 };
 int fd2_debug_math(int fn, const float *x, float *y, int64_t n, int device);
 
@@ -205,6 +229,15 @@ int fd2_get_phs(fd2_ctx *ctx, int geom, int32_t *phs);
 int fd2_get_counters(fd2_ctx *ctx, int geom, uint64_t *counters);
 /* zero all tallies and restart event numbering */
 int fd2_reset(fd2_ctx *ctx);
+
+/* Number the next submitted event `first` (>= 0; events are numbered
+ * consecutively from there).  The number keys each photon's random stream
+ * (64 bits, so streams never repeat within a context), so a job split
+ * across contexts or GPUs gives the same photons as one context when each
+ * part starts at its first global event number.  No counterpart in the
+ * reference, whose CUDA branch reuses its streams in every buffer
+ * (main/kernel_cuda_c_ver1_0.cu:420-424). */
+int fd2_set_next_event(fd2_ctx *ctx, int64_t first);
 
 /* Device memory helpers for callers that keep events resident in HBM
  * (fd2_submit_device) without a GPU runtime of their own. */

@@ -17,6 +17,7 @@
 #include <math.h>
 
 #include "../hybridmantis_amd/csrc/fd2_math.h"
+#include "../hybridmantis_amd/csrc/fd2_ff.h"
 #include <stdlib.h>
 #include <string.h>
 
@@ -102,8 +103,10 @@ typedef struct {
   fdo_tally *t;
   int rng_mode;
   int dev;            /* FDO_MATH_DEVICE: FP32 expressions, fd2_math.h */
+  int emu;            /* FDO_MATH_EMULATED: faithful, double parts by fd2_ff.h */
   int32_t ranecu[2];
   uint32_t key[2];
+  uint32_t event_hi;  /* counter mode: event number >> 32 (counter word 1) */
   uint32_t xs[4];     /* per-photon xoshiro128** state */
 
   const fdo_event *ev;
@@ -137,7 +140,7 @@ static float uni(ctx *c) {
 /* counter mode: photon k of event e draws from a xoshiro128** stream whose
  * 128-bit state is Philox4x32-10(counter = (k, 0, 0, 0), key = (seed, e)). */
 static void counter_photon(ctx *c, uint32_t photon) {
-  uint32_t ctr[4] = {photon, 0u, 0u, 0u};
+  uint32_t ctr[4] = {photon, c->event_hi, 0u, 0u};
   fdo_philox4x32_10(ctr, c->key, c->xs);
   if ((c->xs[0] | c->xs[1] | c->xs[2] | c->xs[3]) == 0u) c->xs[0] = 0x9E3779B9u;
 }
@@ -211,11 +214,19 @@ static void redirect(ctx *c, float t1, float t2, int refract) {
   } else {
     c->t->work[FDO_W_REFRACT]++;
     /* faithful mode only (device mode: refract_dev) */
-    double ct = cos((double)t2);
-    double ratio = sin((double)t2) / sin((double)t1);
-    u[0] = (float)((double)(-n[0]) * ct - ratio * (double)(u[0] + ca * n[0]));
-    u[1] = (float)((double)(-n[1]) * ct - ratio * (double)(u[1] + ca * n[1]));
-    u[2] = (float)((double)(-n[2]) * ct - ratio * (double)(u[2] + ca * n[2]));
+    if (c->emu) {
+      const fd2_ff ct = fd2x_cos(t2);
+      const fd2_ff ratio = fd2x_div(fd2x_sin(t2), fd2x_sin(t1));
+      u[0] = fd2x_refract_component(n[0], u[0] + ca * n[0], ct, ratio);
+      u[1] = fd2x_refract_component(n[1], u[1] + ca * n[1], ct, ratio);
+      u[2] = fd2x_refract_component(n[2], u[2] + ca * n[2], ct, ratio);
+    } else {
+      double ct = cos((double)t2);
+      double ratio = sin((double)t2) / sin((double)t1);
+      u[0] = (float)((double)(-n[0]) * ct - ratio * (double)(u[0] + ca * n[0]));
+      u[1] = (float)((double)(-n[1]) * ct - ratio * (double)(u[1] + ca * n[1]));
+      u[2] = (float)((double)(-n[2]) * ct - ratio * (double)(u[2] + ca * n[2]));
+    }
   }
   renorm(c, u);
 }
@@ -241,6 +252,10 @@ static void rough_n(ctx *c, float *n, float beta, int normalise) {
     p0 = rr * cs;
     p1 = rr * sn;
     nb = 0.0f;
+  } else if (c->emu) {
+    const float th = fd2x_mul_dconst(s * 2.0f, FD2X_PI_D);
+    fd2x_polar(rr, th, &p0, &p1);
+    nb = fd2x_norm3(p0, p1, p2);
   } else {
     float th = (float)((double)(s * 2.0f) * FDO_PI);
     p0 = (float)((double)rr * cos((double)th));
@@ -270,7 +285,7 @@ static void rough_n(ctx *c, float *n, float beta, int normalise) {
     n[1] = r1 * inv;
     n[2] = r2 * inv;
   } else {
-    nb = (float)sqrt(pow(r0, 2) + pow(r1, 2) + pow(r2, 2));
+    nb = c->emu ? fd2x_norm3(r0, r1, r2) : (float)sqrt(pow(r0, 2) + pow(r1, 2) + pow(r2, 2));
     n[0] = r0 / nb;
     n[1] = r1 / nb;
     n[2] = r2 / nb;
@@ -285,6 +300,8 @@ static void azimuth(ctx *c, float rr, float u, float *dx, float *dy) {
     fd2_sincos_turns(u, &sn, &cs);
     *dx = rr * cs;
     *dy = rr * sn;
+  } else if (c->emu) {
+    fd2x_polar(rr, fd2x_mul_dconst(u, FD2X_TWOPI_D), dx, dy);
   } else {
     float th = (float)((double)u * FDO_TWOPI);
     *dx = (float)((double)rr * cos((double)th));
@@ -300,7 +317,9 @@ static void emit_dir(ctx *c) {
   float *u = c->dcos;
   float r = uni(c) * 2.0f - 1.0f;
   u[2] = r;
-  float rr = c->dev ? sqrtf(1.0f - r * r) : (float)sqrt(1.0 - (double)(r * r));
+  float rr = c->dev   ? sqrtf(1.0f - r * r)
+             : c->emu ? fd2x_sqrt_one_minus(r * r)
+                      : (float)sqrt(1.0 - (double)(r * r));
   azimuth(c, rr, uni(c), &u[0], &u[1]);
   renorm(c, u);
 }
@@ -351,6 +370,8 @@ static float dist_to_surface(ctx *c) {
     const float inv2a = 1.0f / (2.0f * qa);
     r1 = (-qb + sqf) * inv2a;
     r2 = (-qb - sqf) * inv2a;
+  } else if (c->emu) {
+    fd2x_roots(qb, (qb * qb) - (4.0f * qa * qc), 2.0f * qa, &r1, &r2);
   } else {
     double sq = sqrt((double)((qb * qb) - (4.0f * qa * qc)));
     r1 = (float)(((double)(-qb) + sq) / (double)(2.0f * qa));
@@ -445,6 +466,8 @@ static int isotropic(ctx *c) {
     c->t->work[FDO_W_ABSLOG]++;
     if (c->dev)
       da = (-1.0f / g->bulk) * fd2_log(uni(c));
+    else if (c->emu)
+      da = fd2x_scaled_log(-1.0f / g->bulk, uni(c));
     else
       da = (float)((double)(-1.0f / g->bulk) * log((double)uni(c)));
   } else {
@@ -507,7 +530,7 @@ static int column_normal(ctx *c, const float *md, int *tries, int always_rough) 
       n[1] = n[1] / den;
     }
     if (always_rough || g->beta > 0.0f) rough(c, n, g->beta);
-    float ang = c->dev ? fd2_acos(dot3(md, n)) : acosf(dot3(md, n));
+    float ang = c->dev ? fd2_acos(dot3(md, n)) : c->emu ? fd2x_acosf(dot3(md, n)) : acosf(dot3(md, n));
     if (ang < 0.0f || ang > 1.57f) {
       if (*tries < 100) {
         (*tries)++;
@@ -689,10 +712,20 @@ static void fresnel_pq(const ctx *c, float t1, float t2, float *pr, float *pt) {
     *pt = 1.0f - *pr;
     return;
   }
+  if (c->emu) {
+    fd2x_fresnel(t1, t2, pr, pt);
+    return;
+  }
   double dm = (double)(t1 - t2), dp = (double)(t1 + t2);
   *pr = (float)(0.5f * (pow(tan(dm), 2) / pow(tan(dp), 2) + pow(sin(dm), 2) / pow(sin(dp), 2)));
   double s12 = sin((double)(2.0f * t1)) * sin((double)(2.0f * t2));
   *pt = (float)(0.5f * (s12 / (pow(sin(dp), 2) * pow(cos(dm), 2)) + s12 / pow(sin(dp), 2)));
+}
+
+/* theta2 = asinf((float)((double)ratio * sin((double)t1))), K:1811-1865 */
+static float snell_t2(const ctx *c, float ratio, float t1) {
+  if (c->emu) return fd2x_asinf(fd2x_scaled_sin(ratio, t1));
+  return asinf((float)((double)ratio * sin((double)t1)));
 }
 
 /* faithful mode (device mode: fresnel_dev) */
@@ -704,7 +737,7 @@ static void fresnel(ctx *c, int inside, float *t1, float *t2, float *pr, float *
       *pt = 0.0f;
     } else {
       if ((double)*t1 < FDO_EPS) *t1 = (float)0.00042;
-      *t2 = asinf((float)((double)(n1 / n2) * sin((double)*t1)));
+      *t2 = snell_t2(c, n1 / n2, *t1);
       c->t->work[FDO_W_FRESNEL]++;
       fresnel_pq(c, *t1, *t2, pr, pt);
     }
@@ -716,11 +749,11 @@ static void fresnel(ctx *c, int inside, float *t1, float *t2, float *pr, float *
       }
     } else if ((double)*t1 < FDO_EPS) {
       *t1 = (float)0.00042;
-      *t2 = asinf((float)((double)(n1 / n2) * sin((double)*t1)));
+      *t2 = snell_t2(c, n1 / n2, *t1);
       c->t->work[FDO_W_FRESNEL]++;
       fresnel_pq(c, *t1, *t2, pr, pt);
     } else {
-      *t2 = asinf((float)((double)(n2 / n1) * sin((double)*t1)));
+      *t2 = snell_t2(c, n2 / n1, *t1);
       c->t->work[FDO_W_FRESNEL]++;
       fresnel_pq(c, *t1, *t2, pr, pt);
     }
@@ -810,9 +843,11 @@ static int boundary_analysis(ctx *c) {
   if (p[2] <= H / 2.0f && p[2] >= (H / 2.0f - depth20))
     cct = 1.0f;
   else if (p[2] < (H / 2.0f - depth20) && p[2] >= 0.0f)
-    cct = (float)((double)(p[2] / (2.0f * (H / 2.0f - depth20))) + 0.5);
+    cct = c->emu ? p[2] / (2.0f * (H / 2.0f - depth20)) + 0.5f
+                 : (float)((double)(p[2] / (2.0f * (H / 2.0f - depth20))) + 0.5);
   else if (p[2] < 0.0f && p[2] >= (-H / 2.0f))
-    cct = (float)((double)((p[2] - (-H / 2.0f)) / (2.0f * (-H / 2.0f))) + 1.0);
+    cct = c->emu ? (p[2] - (-H / 2.0f)) / (2.0f * (-H / 2.0f)) + 1.0f
+                 : (float)((double)((p[2] - (-H / 2.0f)) / (2.0f * (-H / 2.0f))) + 1.0);
 
   if (uni(c) < cct) {
     /* K:2452-2488 transmit with flagCCT = 1: the neighbour column gets a
@@ -888,7 +923,7 @@ no_perturbation:  /* K:1746 */
   if (c->dev) {
     fresnel_dev(c, inside, t1, &s1, &s2, &c2, &pr);
   } else {
-    t1 = acosf(t1);
+    t1 = c->emu ? fd2x_acosf(t1) : acosf(t1);
     fresnel(c, inside, &t1, &t2, &pr, &pt);
     float tot = pr + pt;
     pr = pr / tot;
@@ -898,7 +933,7 @@ no_perturbation:  /* K:1746 */
   if (uni(c) < pr) {  /* reflection, K:1874-1932 */
     redirect(c, t1, t2, 0);
     c->t->work[FDO_W_REFLCHK]++;
-    float ang = c->dev ? fd2_acos(dot3(on, u)) : acosf(dot3(on, u));
+    float ang = c->dev ? fd2_acos(dot3(on, u)) : c->emu ? fd2x_acosf(dot3(on, u)) : acosf(dot3(on, u));
     if (ang > 1.57f) {
       reperturb++;
       if (reperturb < 4) goto reperturb;
@@ -945,6 +980,7 @@ static void run_event(ctx *c, const fdo_event *ev, int64_t gid, uint32_t seed,
   } else {
     c->key[0] = seed;
     c->key[1] = (uint32_t)gid;
+    c->event_hi = (uint32_t)((uint64_t)gid >> 32);
   }
   int first_done = 0;
   for (int32_t k = 0; k < ev->N; ++k) {
@@ -1014,7 +1050,9 @@ int fdo_run(const fdo_geom *g, const fdo_event *events, int64_t n,
             int64_t event_base, int rng_mode, int math_mode, uint32_t seed,
             fdo_tally *t) {
   if (!g || !t || (n > 0 && !events)) return -1;
-  if (math_mode != FDO_MATH_FAITHFUL && math_mode != FDO_MATH_DEVICE) return -1;
+  if (math_mode != FDO_MATH_FAITHFUL && math_mode != FDO_MATH_DEVICE &&
+      math_mode != FDO_MATH_EMULATED)
+    return -1;
   if (rng_mode != FDO_RNG_RANECU && rng_mode != FDO_RNG_COUNTER) return -1;
   ctx c;
   memset(&c, 0, sizeof c);
@@ -1022,6 +1060,7 @@ int fdo_run(const fdo_geom *g, const fdo_event *events, int64_t n,
   c.t = t;
   c.rng_mode = rng_mode;
   c.dev = math_mode == FDO_MATH_DEVICE;
+  c.emu = math_mode == FDO_MATH_EMULATED;
   fdo_photon *rec = t->photons;
   for (int64_t i = 0; i < n; ++i) {
     run_event(&c, &events[i], event_base + i, seed, rec);
@@ -1148,7 +1187,7 @@ int fdo_retrace_step_f64(const float pin[3], const float dc[3], float xcf,
 /* ------------------------------------------------------- test hooks */
 
 /* n uniforms of photon `photon`'s counter-mode stream */
-void fdo_counter_stream(uint32_t seed, uint32_t event, uint32_t photon,
+void fdo_counter_stream(uint32_t seed, uint64_t event, uint32_t photon,
                         int64_t n, float *out) {
   fdo_tally t;
   memset(&t, 0, sizeof t);
@@ -1157,7 +1196,8 @@ void fdo_counter_stream(uint32_t seed, uint32_t event, uint32_t photon,
   c.t = &t;
   c.rng_mode = FDO_RNG_COUNTER;
   c.key[0] = seed;
-  c.key[1] = event;
+  c.key[1] = (uint32_t)event;
+  c.event_hi = (uint32_t)(event >> 32);
   counter_photon(&c, photon);
   for (int64_t i = 0; i < n; ++i) out[i] = uni(&c);
 }
@@ -1180,6 +1220,12 @@ int fdo_math_eval(int fn, const float *x, float *y, int64_t n) {
         y[i] = (a < 0.0f || a > 1.57f) ? 1.0f : 0.0f;
         break;
       }
+      /* the faithful arithmetic's pieces (fd2_ff.h) */
+      case 10: y[i] = fd2x_acosf(x[i]); break;
+      case 11: y[i] = fd2x_asinf(x[i]); break;
+      case 12: y[i] = fd2x_sin(x[i]).hi; break;
+      case 13: y[i] = fd2x_cos(x[i]).hi; break;
+      case 14: y[i] = fd2x_log(x[i]).hi; break;
       default: return -1;
     }
   }
@@ -1191,6 +1237,7 @@ void fdo_fresnel_probs(float t1, float t2, int math_mode, float *pr, float *pt) 
   ctx c;
   memset(&c, 0, sizeof c);
   c.dev = math_mode == FDO_MATH_DEVICE;
+  c.emu = math_mode == FDO_MATH_EMULATED;
   fresnel_pq(&c, t1, t2, pr, pt);
 }
 
@@ -1204,6 +1251,7 @@ void fdo_emit_dirs(uint32_t seed, int math_mode, int64_t n, float *out3) {
   c.t = &t;
   c.rng_mode = FDO_RNG_COUNTER;
   c.dev = math_mode == FDO_MATH_DEVICE;
+  c.emu = math_mode == FDO_MATH_EMULATED;
   c.key[0] = seed;
   c.key[1] = 0;
   for (int64_t i = 0; i < n; ++i) {
@@ -1225,6 +1273,7 @@ void fdo_rough_normals(uint32_t seed, int math_mode, float beta, const float n0[
   c.t = &t;
   c.rng_mode = FDO_RNG_COUNTER;
   c.dev = math_mode == FDO_MATH_DEVICE;
+  c.emu = math_mode == FDO_MATH_EMULATED;
   c.key[0] = seed;
   c.key[1] = 1;
   for (int64_t i = 0; i < n; ++i) {

@@ -17,9 +17,10 @@
  *                   (init_PRNG(event_index, 50000, seed_input)); the photons
  *                   of an event consume it in order.
  *   FDO_RNG_COUNTER one stream per photon: xoshiro128** whose 128-bit
- *                   state is Philox4x32-10(counter = (photon, 0, 0, 0),
- *                   key = (seed, event id)) -- the production kernel's
- *                   stream layout, independent of scheduling.
+ *                   state is Philox4x32-10(counter = (photon, event id
+ *                   >> 32, 0, 0), key = (seed, event id mod 2^32)) -- the
+ *                   production kernel's stream layout, independent of
+ *                   scheduling, distinct for all 64-bit event ids.
  */
 #ifndef FDO_H
 #define FDO_H
@@ -43,8 +44,13 @@ enum { FDO_RNG_RANECU = 0, FDO_RNG_COUNTER = 1 };
  *                      for normalisations, the new-column normal's xy
  *                      projection taken before normalising, and the
  *                      Fresnel branch from the sines and cosines of the
- *                      angles (Snell's law) instead of the angles. */
-enum { FDO_MATH_FAITHFUL = 0, FDO_MATH_DEVICE = 1 };
+ *                      angles (Snell's law) instead of the angles;
+ *   FDO_MATH_EMULATED  FAITHFUL with each double expression evaluated by
+ *                      hybridmantis_amd/csrc/fd2_ff.h (float pairs, no
+ *                      double) and the C library's acosf/asinf by their
+ *                      restatements there: the GPU's faithful trace mode
+ *                      restated on the CPU. */
+enum { FDO_MATH_FAITHFUL = 0, FDO_MATH_DEVICE = 1, FDO_MATH_EMULATED = 2 };
 
 /* photon fates (exactly one per transported photon) */
 enum {
@@ -118,7 +124,8 @@ typedef struct {
 } fdo_tally;
 
 /* Transport events [0, n).  event_base is the global id of events[0] (used
- * as the Philox key and as the RANECU history_batch index). */
+ * as the Philox key -- low word in the key, high word in counter word 1 --
+ * and as the RANECU history_batch index). */
 int fdo_run(const fdo_geom *g, const fdo_event *events, int64_t n,
             int64_t event_base, int rng_mode, int math_mode, uint32_t seed,
             fdo_tally *t);
@@ -146,7 +153,7 @@ int fdo_retrace_step_f64(const float pos_in[3], const float dcos[3], float xc,
                          double pos_out[3]);
 
 /* Test hooks (closed-form and distribution checks). */
-void fdo_counter_stream(uint32_t seed, uint32_t event, uint32_t photon,
+void fdo_counter_stream(uint32_t seed, uint64_t event, uint32_t photon,
                         int64_t n, float *out);
 int fdo_math_eval(int fn, const float *x, float *y, int64_t n);
 void fdo_fresnel_probs(float t1, float t2, int math_mode, float *pr, float *pt);

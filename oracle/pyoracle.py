@@ -21,6 +21,7 @@ RNG_RANECU = 0
 RNG_COUNTER = 1
 MATH_FAITHFUL = 0
 MATH_DEVICE = 1
+MATH_EMULATED = 2
 
 FATE_NONE, FATE_DETECT, FATE_DETECT_OUT, FATE_ABS_TOP, FATE_ABS_BULK, \
     FATE_LOST, FATE_OUTOFCOL, FATE_THETA1 = range(8)
@@ -243,14 +244,15 @@ def philox(ctr, key):
 def counter_stream(seed: int, event: int, photon: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.float32)
     f = lib().fdo_counter_stream
-    f.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int64, C.c_void_p]
-    f(seed & 0xFFFFFFFF, event & 0xFFFFFFFF, photon & 0xFFFFFFFF, n, out.ctypes.data)
+    f.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_int64, C.c_void_p]
+    f(seed & 0xFFFFFFFF, event & 0xFFFFFFFFFFFFFFFF, photon & 0xFFFFFFFF, n, out.ctypes.data)
     return out
 
 
 MATH_FN = {"sin": 0, "cos": 1, "log": 2, "asin": 3, "acos": 4,
            "sin_turns": 5, "cos_turns": 6, "rsqrt": 7, "sqrt_unit": 8,
-           "angle_over_157": 9}
+           "angle_over_157": 9,
+           "acosf_libc": 10, "asinf_libc": 11, "ff_sin": 12, "ff_cos": 13, "ff_log": 14}
 
 
 def math_eval(fn: str, x) -> np.ndarray:
@@ -306,22 +308,24 @@ def retrace_step(pos, dcos, xc, yc, p: dict, forced: int = -1):
 # ---------------------------------------------------------- multi-process
 
 def _slice_worker(args):
-    kind, p, ev, num_hist, rng, seed, base, math, rezero = args
+    kind, p, ev, num_hist, rng, seed, base, math, rezero, record = args
     import time as _t
     t0 = _t.perf_counter()
     if kind == "ref":
-        r = run_ref(p, ev, num_hist, seed=seed, event_base=base, rezero=rezero)
+        r = run_ref(p, ev, num_hist, seed=seed, event_base=base, rezero=rezero,
+                    record=record)
         work = None
     else:
-        r = run(p, ev, num_hist, rng=rng, seed=seed, event_base=base, math=math)
+        r = run(p, ev, num_hist, rng=rng, seed=seed, event_base=base, math=math,
+                record=record)
         work = r.work
-    return (r.prf, r.hist_count, r.counters, work, _t.perf_counter() - t0)
+    return (r.prf, r.hist_count, r.counters, work, _t.perf_counter() - t0, r.photons)
 
 
 def run_parallel(p: dict, events, num_hist: int, workers: int = 8,
                  kind: str = "oracle", rng: int = RNG_COUNTER,
                  seed: int = 271828182, math: int = MATH_FAITHFUL,
-                 rezero: bool = False, timeout: float = 600.0):
+                 rezero: bool = False, timeout: float = 600.0, record: bool = False):
     """Run contiguous event slices in separate processes (the transport
     keeps per-process state, like the reference C path) and sum the
     tallies; results equal a single-process run because streams are keyed
@@ -334,7 +338,7 @@ def run_parallel(p: dict, events, num_hist: int, workers: int = 8,
     workers = max(1, min(workers, n))
     bounds = [n * i // workers for i in range(workers + 1)]
     jobs = [(kind, p, ev[bounds[i]:bounds[i + 1]].copy(), num_hist, rng, seed,
-             bounds[i], math, rezero) for i in range(workers)]
+             bounds[i], math, rezero, record) for i in range(workers)]
     # fresh interpreters (spawn): the caller may hold a GPU context; a dead
     # worker raises BrokenProcessPool instead of hanging
     with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
@@ -344,4 +348,5 @@ def run_parallel(p: dict, events, num_hist: int, workers: int = 8,
     hc = sum(o[1].astype(np.uint64) for o in outs).astype(np.uint32)
     cnt = sum(o[2] for o in outs)
     work = None if outs[0][3] is None else sum(o[3] for o in outs)
-    return Result(prf, hc, cnt, work, None), [o[4] for o in outs]
+    photons = np.concatenate([o[5] for o in outs]) if record else None
+    return Result(prf, hc, cnt, work, photons), [o[4] for o in outs]

@@ -1,6 +1,9 @@
 """The transport's device arithmetic on the GPU (fd2_debug_math) against the
 same functions on the CPU (the oracle's fdo_math_eval), bit for bit, and the
-GPU square root against the correctly rounded float32 square root."""
+GPU square root against the correctly rounded float32 square root.  The
+faithful arithmetic's pieces (restated C-library acosf/asinf, float-pair
+sin/cos/log) likewise: the CPU side of those is pinned to the C library and
+to the double functions by oracle/check_faithful.c."""
 import numpy as np
 import pytest
 
@@ -10,6 +13,8 @@ FNS = {  # argument ranges the transport passes
     "sin": (-3.2, 6.4), "cos": (-3.2, 6.4), "log": (6e-8, 1.0), "asin": (-1.0, 1.0),
     "acos": (-1.0, 1.0), "sin_turns": (0.0, 1.0), "cos_turns": (0.0, 1.0),
     "rsqrt": (1e-6, 1e6), "sqrt_unit": (0.0, 4.0), "angle_over_157": (-1.0, 1.0),
+    "acosf_libc": (-1.0, 1.0), "asinf_libc": (-1.0, 1.0), "ff_sin": (-8.0, 8.0),
+    "ff_cos": (-8.0, 8.0), "ff_log": (4.6e-10, 1.0),
 }
 
 
@@ -59,3 +64,21 @@ def test_sqrt_unit_correctly_rounded(fd2lib):
     y = fd2.debug_math("sqrt_unit", x)
     assert np.array_equal(y.view(np.uint32), np.sqrt(x).view(np.uint32))
     assert np.isnan(fd2.debug_math("sqrt_unit", np.array([-1.0, np.nan], np.float32))).all()
+
+
+def test_libc_arc_functions_every_float_in_two_binades(fd2lib, oracle):
+    """The restated acosf/asinf on the GPU against the CPU restatement for
+    every float in [0.5, 1] and [-1, -0.5] (their half-angle branches, with
+    the |x| > 0.975 branch of asinf) and every 64th float of (-0.5, 0.5)."""
+    from hybridmantis_amd import fd2
+
+    h = np.float32(0.5).view(np.uint32)
+    one = np.float32(1.0).view(np.uint32)
+    top = np.arange(h, one + 1, dtype=np.uint32)
+    mid = np.arange(0, h, 64, dtype=np.uint32)
+    bits = np.concatenate([top, top | np.uint32(0x80000000), mid, mid | np.uint32(0x80000000)])
+    x = bits.view(np.float32)
+    for fn in ("acosf_libc", "asinf_libc"):
+        print(f"[gpu-math] {fn} n={len(x)}", flush=True)
+        g, c = fd2.debug_math(fn, x), oracle.math_eval(fn, x)
+        assert np.array_equal(g.view(np.uint32), c.view(np.uint32)), fn

@@ -1,5 +1,14 @@
 """GPU transport (libfd2, HIP on gfx950) against the oracle.
 
+* Trajectory test against the reference itself: the trajectory kernel in
+  the reference C path's own arithmetic (fd2_trace_ranecu_faithful: float
+  state, the double expressions evaluated with float pairs, the C library's
+  acosf/asinf restated; no FP64 on the GPU) on the reference's per-event
+  RANECU streams reproduces the compiled reference's fates and pixels
+  (tests/golden/ref_*.npz) and the oracle's faithful mode on the full
+  2,000-history case; at most 0.1% of photons may differ (a divergence in a
+  RANECU event shifts the rest of that event's stream, so only the first
+  differing photon of an event is counted).
 * Trajectory tests: the GPU and the oracle's FP32 restatement
   (MATH_DEVICE) consume the same uniform streams -- the reference's
   per-event RANECU streams (one lane per event) and the production kernel's
@@ -52,6 +61,71 @@ def first_divergences(events, a, b):
         else:
             compared += int(n[e])
     return bad, compared
+
+
+def load_reference_fixture(golden, name):
+    here, man = golden
+    d = np.load(os.path.join(here, man[name]["file"]))
+    prf = np.zeros(int(np.prod(d["prf_shape"])), np.uint64)
+    prf[d["prf_idx"]] = d["prf_val"]
+    return d, prf.reshape(tuple(d["prf_shape"]))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_trajectory_faithful_matches_reference(fd2lib, oracle, golden, name):
+    """north_star's trajectory test against the reference itself: same
+    RANECU streams, the reference's arithmetic; fates and pixels of every
+    photon against what the reference's compiled C branch produced."""
+    p, ev, seed = load_case(golden, name)
+    d, ref_prf = load_reference_fixture(golden, name)
+    print(f"[trajectory-faithful] {name}: {len(ev)} events, {int(ev['N'].sum())} photons",
+          flush=True)
+    with fd2lib.Transport(p) as tr:
+        rec = tr.trace_ranecu(ev, seed_input=seed, faithful=True)
+        counters = tr.counters()
+        prf, hist = tr.prf(), tr.hist_counts()
+    ref = {"fate": d["fate"].astype(np.int32), "pix": d["pix"]}
+    bad, compared = first_divergences(ev, rec, ref)
+    print(f"  events diverging {bad}, photons compared {compared}", flush=True)
+    assert bad <= 0.001 * compared, (bad, compared)
+    if bad == 0:
+        assert np.array_equal(prf, ref_prf)
+        assert np.array_equal(hist, d["hist"])
+        assert [counters[k] for k in oracle.COUNTER_NAMES[:7]] == d["counters"][:7].tolist()
+    # the GPU computes exactly the oracle's emulated arithmetic, bit for bit
+    o = oracle.run(p, ev, p["num_hist"], rng=oracle.RNG_RANECU, seed=seed,
+                   record=True, math=oracle.MATH_EMULATED)
+    for f in ("fate", "pix", "x", "y"):
+        assert np.array_equal(rec[f], o.photons[f]), f
+
+
+def test_trajectory_faithful_full_reference_case(fd2lib, oracle):
+    """The reference's CPU-runnable case in full (2,000 histories at 25 keV,
+    ~2.2e6 photons, 501x501 1-um PRF) on RANECU streams: the faithful GPU
+    kernel against the oracle's faithful mode (the reference's float/double
+    arithmetic with the C library, bit-exact against the compiled reference
+    on the fixtures), photon by photon.  First-divergence bar 0.1%; the
+    float-pair evaluation rounds like double except within ~2^-46 of a
+    rounding boundary, measured at 2 events in 2.2e6 photons on the CPU."""
+    p, ev = E.workload("mono25", scale=1.0)
+    seed = 123456789
+    print(f"[trajectory-faithful] mono25 config 0: {len(ev)} events "
+          f"{int(ev['N'].sum())} photons", flush=True)
+    with fd2lib.Transport(p) as tr:
+        rec = tr.trace_ranecu(ev, seed_input=seed, faithful=True)
+        prf, counters = tr.prf(), tr.counters()
+    o, _ = oracle.run_parallel(p, ev, p["num_hist"], workers=min(16, os.cpu_count() or 1),
+                               rng=oracle.RNG_RANECU, seed=seed, math=oracle.MATH_FAITHFUL,
+                               record=True)
+    bad, compared = first_divergences(ev, rec, o.photons)
+    print(f"  events diverging {bad}, photons compared {compared}", flush=True)
+    assert bad <= 0.001 * compared, (bad, compared)
+    assert counters["generated"] == int(ev["N"].sum())
+    # tallies agree up to the few diverging events' photons
+    n_det = int(o.counters[1])
+    assert abs(counters["detected"] - n_det) <= 2 * int(ev["N"][:].max()) * bad
+    if bad == 0:
+        assert np.array_equal(prf, o.prf)
 
 
 @pytest.mark.parametrize("name", CASES)

@@ -80,3 +80,22 @@ def test_angle_and_sqrt_unit_oracle(oracle):
     assert list(a[6:]) == [0.0, 0.0, 0.0]  # NaN angles pass the test
     s = np.array([0.0, 2.0 ** -24, 0.25, 1.0, np.inf], np.float32)
     assert np.array_equal(oracle.math_eval("sqrt_unit", s), np.sqrt(s))
+
+
+def test_faithful_arithmetic_against_c_library():
+    """oracle/check_faithful.c: the restated acosf/asinf equal the C
+    library's for every float in [-1, 1]; the float-pair sin/cos/log are
+    within 2^-45 of the double functions; each double expression of the
+    reference, evaluated with float pairs, rounds to the same float as its
+    double evaluation (at most 16 of 2^22 random arguments may differ; the
+    expected rate is about 1e-6); the crosstalk pieces equal float sums."""
+    import os
+    import subprocess
+    here = os.path.join(os.path.dirname(__file__), "..", "oracle")
+    subprocess.run(["make", "-s", "-C", here, "check_faithful"], check=True)
+    env = dict(os.environ, OMP_NUM_THREADS=str(min(8, os.cpu_count() or 1)))
+    r = subprocess.run([os.path.join(here, "check_faithful")], capture_output=True, text=True,
+                       env=env, timeout=900)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("violations 0") == 5, r.stdout

@@ -38,6 +38,24 @@ def test_oracle_matches_reference_fixture(oracle, golden, name):
     assert np.array_equal(r.counters[:7], d["counters"][:7])
 
 
+@pytest.mark.parametrize("name", CASES)
+def test_emulated_oracle_matches_reference_fixture(oracle, golden, name):
+    """The faithful arithmetic without double precision (fd2_ff.h: float
+    pairs for the double expressions, the C library's acosf/asinf restated)
+    reproduces the reference's own trajectories: every photon's fate and
+    pixel, the PRF, the per-history counts and the counters.  The GPU's
+    faithful trace kernel computes exactly this (tests/test_gpu_transport.py)."""
+    m, d, prf = load(golden, name)
+    p = m["params"]
+    r = oracle.run(p, d["events"], p["num_hist"], rng=oracle.RNG_RANECU,
+                   seed=m["seed"], record=True, math=oracle.MATH_EMULATED)
+    assert np.array_equal(r.photons["fate"], d["fate"].astype(np.int32))
+    assert np.array_equal(r.photons["pix"], d["pix"])
+    assert np.array_equal(r.prf, prf)
+    assert np.array_equal(r.hist_count, d["hist"])
+    assert np.array_equal(r.counters[:7], d["counters"][:7])
+
+
 def test_fixture_counters_are_consistent(golden):
     # each photon ends exactly once: generated = sum of the fate counters
     for name in CASES:
@@ -75,3 +93,7 @@ def test_oracle_matches_reference_live(oracle):
     assert np.array_equal(r.photons["fate"], o.photons["fate"])
     assert np.array_equal(r.photons["pix"], o.photons["pix"])
     assert np.array_equal(r.prf, o.prf)
+    e = oracle.run(p, ev, 30, rng=oracle.RNG_RANECU, seed=987654321, record=True,
+                   math=oracle.MATH_EMULATED)
+    assert np.array_equal(r.photons["fate"], e.photons["fate"])
+    assert np.array_equal(r.photons["pix"], e.photons["pix"])

@@ -86,3 +86,39 @@ def test_counter_streams_uniform_and_independent(oracle):
     assert not np.array_equal(a[:16], b[:16])
     # 24-bit resolution: values are multiples of 2^-24
     assert np.all((u * 2**24) == np.round(u * 2**24))
+
+
+def _xoshiro_u(state, n):
+    """xoshiro128** (Blackman & Vigna 2018) from a 4-word state, 24 high
+    bits -> (0, 1], in Python integers."""
+    s = [int(v) for v in state]
+    if not any(s):
+        s[0] = 0x9E3779B9
+    m = 0xFFFFFFFF
+    out = []
+    for _ in range(n):
+        x = (s[1] * 5) & m
+        r = ((((x << 7) | (x >> 25)) & m) * 9) & m
+        t = (s[1] << 9) & m
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = ((s[3] << 11) | (s[3] >> 21)) & m
+        out.append(np.float32(((r >> 8) + 1) / 16777216.0))
+    return np.array(out, np.float32)
+
+
+@pytest.mark.parametrize("event", [0, 5, 2**32 - 1, 2**32, 2**32 + 5, 3 * 2**32 + 77, 2**63 + 1])
+def test_counter_stream_key_is_64_bit(oracle, event):
+    """A photon's stream state is Philox4x32-10(counter (photon, event >>
+    32, 0, 0), key (seed, event mod 2^32)): below 2^32 the high word is 0
+    (the 32-bit keying), and events 2^32 apart get different streams."""
+    seed, photon = 271828182, 9
+    want = _xoshiro_u(oracle.philox([photon, event >> 32, 0, 0], [seed, event & 0xFFFFFFFF]), 16)
+    got = oracle.counter_stream(seed, event, photon, 16)
+    assert np.array_equal(got, want)
+    if event >= 2**32:
+        low = oracle.counter_stream(seed, event & 0xFFFFFFFF, photon, 16)
+        assert not np.array_equal(got, low)
