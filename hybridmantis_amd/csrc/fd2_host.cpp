@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "fd2_internal.h"
@@ -191,8 +192,16 @@ struct Launch {
   bool multi = false, record = false, radial = false;
 };
 
+// Queue cursor and error flag of a launch other than the context's own,
+// with the transport in park-in-place mode (no per-wave queues).
+struct NoQueueScratch {
+  uint32_t* cursor;
+  uint32_t* error_flag;
+};
+
 Launch make_launch(const fd2_ctx* c, const QueueBufs& b, const fd2_event* d_ev, int64_t n,
-                   fd2_photon* d_records, uint32_t* d_radial) {
+                   fd2_photon* d_records, uint32_t* d_radial,
+                   const NoQueueScratch* own = nullptr) {
   Launch L;
   QueueArgs& q = L.q;
   q.events = d_ev;
@@ -200,8 +209,8 @@ Launch make_launch(const fd2_ctx* c, const QueueBufs& b, const fd2_event* d_ev, 
   q.photon_off = b.photon_off;
   q.block_chunks = b.block_chunks;
   q.block_photons = b.block_photons;
-  q.cursor = c->d_cursor;
-  q.error_flag = c->d_error;
+  q.cursor = own ? own->cursor : c->d_cursor;
+  q.error_flag = own ? own->error_flag : c->d_error;
   q.n_events = n;
   q.n_geoms = c->n_geoms;
   q.chunk_log2 = ilog2(c->tune.chunk);
@@ -214,13 +223,13 @@ Launch make_launch(const fd2_ctx* c, const QueueBufs& b, const fd2_event* d_ev, 
   a.photon_off = b.photon_off;
   a.geoms = c->d_geoms;
   a.counters = c->d_counters;
-  a.cursor = c->d_cursor;
+  a.cursor = q.cursor;
   a.records = d_records;
   a.radial = d_radial;
   a.event_base_dev = nullptr;
-  const bool queue = c->tune.wall_queue >= 0 && c->d_wave_queue != nullptr;
+  const bool queue = !own && c->tune.wall_queue >= 0 && c->d_wave_queue != nullptr;
   a.wave_queue = queue ? c->d_wave_queue : nullptr;
-  a.error_flag = c->d_error;
+  a.error_flag = q.error_flag;
   a.n_events = n;
   a.seed = c->seed;
   a.event_base = (uint64_t)c->next_event;
@@ -304,6 +313,47 @@ int check_error_flag(fd2_ctx* c) {
   return FD2_OK;
 }
 
+// ------------------------------------------- allocation-free submit scratch
+//
+// fd2_submit_device_noalloc keeps its work-queue scratch -- cursor, error
+// flag, block totals and the per-event chunk and photon offsets of at most
+// FD2_NOALLOC_MAX_EVENTS events -- in this file-scope device array, 12.6 MB
+// that each GPU the library runs on holds until the process ends.  Host
+// code uses the array only through the address hipGetSymbolAddress gives
+// on each device (its own copy there), looked up once per device at
+// fd2_create with that device current.
+constexpr size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+constexpr int64_t kNaEvents = FD2_NOALLOC_MAX_EVENTS;
+constexpr int64_t kNaBlocks = (kNaEvents + kScanTile - 1) / kScanTile;
+constexpr size_t kNaBlockChunks = 256;  // [0, 256): cursor, error flag
+constexpr size_t kNaBlockPhotons = align256(kNaBlockChunks + sizeof(uint32_t) * kNaBlocks);
+constexpr size_t kNaChunkOff = align256(kNaBlockPhotons + sizeof(unsigned long long) * kNaBlocks);
+constexpr size_t kNaPhotonOff = align256(kNaChunkOff + sizeof(uint32_t) * (kNaEvents + 1));
+constexpr size_t kNaBytes = align256(kNaPhotonOff + sizeof(unsigned long long) * (kNaEvents + 1));
+constexpr int kNaMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kNaMaxDevices] = {};
+
+// the device address of `scratch` on `device` (current), looked up once
+int scratch_setup(int device) {
+  if (device < 0 || device >= kNaMaxDevices) return FD2_OK;  // that device: no noalloc
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return FD2_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FD2_E_DEVICE;
+  g_scratch_dev[device] = (unsigned char*)p;
+  return FD2_OK;
+}
+
+unsigned char* scratch_of(int device) {
+  if (device < 0 || device >= kNaMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_dev[device];
+}
+
 }  // namespace
 
 namespace fd2 {
@@ -367,6 +417,7 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
   int rc = FD2_OK;
   do {
     if (hipSetDevice(device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
+    if ((rc = scratch_setup(device)) != FD2_OK) break;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
     c->n_cu = prop.multiProcessorCount;
@@ -532,6 +583,33 @@ int fd2_submit_device(fd2_ctx* c, const fd2_event* d_events, int64_t n) {
   FD2_HIP(hipSetDevice(c->device));
   return enqueue(c, d_events, n, false, nullptr);
 }
+
+int fd2_submit_device_noalloc(fd2_ctx* c, const fd2_event* d_events, int64_t n) {
+  if (!c || n < 0 || (n > 0 && !d_events)) return FD2_E_INVAL;
+  if (n > FD2_NOALLOC_MAX_EVENTS) return FD2_E_INVAL;
+  unsigned char* s = scratch_of(c->device);
+  if (!s) return FD2_E_STATE;
+  if (n == 0) return FD2_OK;
+  FD2_HIP(hipSetDevice(c->device));
+  const NoQueueScratch own{(uint32_t*)s, (uint32_t*)s + 1};
+  const QueueBufs b{(uint32_t*)(s + kNaChunkOff), (unsigned long long*)(s + kNaPhotonOff),
+                    (uint32_t*)(s + kNaBlockChunks), (unsigned long long*)(s + kNaBlockPhotons)};
+  const Launch L = make_launch(c, b, d_events, n, nullptr, nullptr, &own);
+  // the array keeps the last call's contents: the error flag starts at zero
+  // here, the queue build zeroes the cursor, and the offsets are rewritten
+  FD2_HIP(launch_fill_u32(own.error_flag, 1, c->stream));
+  FD2_HIP(hipEventRecord(c->ev_start, c->stream));
+  FD2_HIP(issue(L, c->stream));
+  FD2_HIP(hipEventRecord(c->ev_stop, c->stream));
+  c->timed = true;
+  c->next_event += n;
+  uint32_t flag = 0;
+  FD2_HIP(hipMemcpyAsync(&flag, own.error_flag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
+  FD2_HIP(hipStreamSynchronize(c->stream));
+  if (flag) return (flag & 2u) ? FD2_E_DEVICE : FD2_E_INVAL;
+  return FD2_OK;
+}
+
 
 int fd2_sync(fd2_ctx* c) {
   if (!c) return FD2_E_INVAL;

@@ -102,6 +102,7 @@ def lib() -> C.CDLL:
         "fd2_trace_ranecu": ([P, P, C.c_int64, C.c_int64, C.c_int32, P], C.c_int),
         "fd2_trace_ranecu_faithful": ([P, P, C.c_int64, C.c_int64, C.c_int32, P], C.c_int),
         "fd2_set_next_event": ([P, C.c_int64], C.c_int),
+        "fd2_submit_device_noalloc": ([P, P, C.c_int64], C.c_int),
         "fd2_debug_steps": ([P, P, C.c_int64, C.c_int32, P, C.c_int32,
                              C.POINTER(C.c_int32)], C.c_int),
         "fd2_debug_math": ([C.c_int, P, P, C.c_int64, C.c_int], C.c_int),
@@ -283,6 +284,11 @@ class Transport:
 
     def submit_device(self, buf: DeviceBuffer, n: int) -> None:
         _check(lib().fd2_submit_device(self._ctx, buf.ptr, n), "fd2_submit_device")
+
+    def submit_device_noalloc(self, buf: DeviceBuffer, n: int) -> None:
+        """fd2_submit_device_noalloc: synchronous, allocates nothing."""
+        _check(lib().fd2_submit_device_noalloc(self._ctx, buf.ptr, n),
+               "fd2_submit_device_noalloc")
 
     def sync(self) -> None:
         _check(lib().fd2_sync(self._ctx), "fd2_sync")

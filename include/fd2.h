@@ -153,6 +153,24 @@ int fd2_submit(fd2_ctx *ctx, const fd2_event *events, int64_t n);
 int fd2_submit_device(fd2_ctx *ctx, const fd2_event *d_events, int64_t n);
 int fd2_sync(fd2_ctx *ctx);
 
+/* fd2_submit_device without any allocation, for callers that cannot call
+ * hipMalloc (code under stream/graph capture, tight serving loops,
+ * harnesses that forbid it).  No counterpart in the reference.  The work
+ * queue's scratch (cursor, error flag, block totals, per-event chunk and
+ * photon offsets) lives in a static device array of the library sized for
+ * FD2_NOALLOC_MAX_EVENTS events (12.6 MB per GPU, held until the process
+ * ends); the transport runs in park-in-place mode (no per-wave photon
+ * queues), which gives the same photons as fd2_submit_device.  The events
+ * are in device memory (d_events, valid for the call); the tallies
+ * accumulate in the context as for every submit, and the events take the
+ * next n event numbers.  Synchronous: returns the launch's status
+ * (FD2_E_INVAL for n > FD2_NOALLOC_MAX_EVENTS, an invalid event or more work
+ * items than tune.max_chunks).  Every call on one GPU shares that GPU's
+ * scratch array: calls that overlap on one GPU (from several host threads
+ * or contexts) race and must be serialised by the caller. */
+#define FD2_NOALLOC_MAX_EVENTS 1048576
+int fd2_submit_device_noalloc(fd2_ctx *ctx, const fd2_event *d_events, int64_t n);
+
 /* Same as fd2_submit but synchronous, and writes one record per photon
  * (Σ N records, events in order, photons in order). */
 int fd2_submit_record(fd2_ctx *ctx, const fd2_event *events, int64_t n,
