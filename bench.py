@@ -11,9 +11,9 @@ the persistent transport kernel, events already resident in HBM.  The JSON
 line reports optical photons/s over K timed steps, the FP32-VALU roofline of
 the transport kernel (algorithmic FLOPs from hybridmantis_amd/flops.py with
 per-photon primitive counts measured by the oracle on a slice of the same
-workload) and a CPU baseline: the reference's own C transport branch
-(oracle/_ref, or the oracle restatement when that is not built) run on a
-bounded sample in one process per host core.
+workload) and a CPU baseline: the project's restatement of the reference C
+path (oracle/, the C path's own arithmetic and random streams) run on a
+bounded sample in one process per host core, built -O2 and -O0.
 
 Two more legs go into the same line: "batched" (4 geometries x 15-90 keV,
 50,000 histories each, autotuned launch shape, HIP-graph replays) and
@@ -37,35 +37,67 @@ METRIC = "optical photons/s (plain MC); histories/s surrogate vs MC at <=2% PRF 
 FP32_PEAK_TFLOPS = 157.3
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    """CPU cores this process may use; the one-GPU box allots 16 per GPU."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(p, ev, workers: int, photons_per_worker: int):
-    """Reference C branch (or the oracle) on the first events of the
-    workload; ~photons_per_worker photons per process."""
+    """The project's restatement of the reference C path (oracle/fdo.c in
+    the C path's own float/double arithmetic on its per-event RANECU
+    streams: main/hybridMANTIS_c_ver1_0.c driver + the C branch of
+    main/kernel_cuda_c_ver1_0.cu; bit-identical to the compiled reference on
+    the golden fixtures) on the first events of the workload, one process
+    per core over contiguous slices, ~photons_per_worker photons each.
+    Timed built -O2 (the value) and -O0 (how the reference builds it,
+    main/compile_ver1_0.sh:9-11), each with and without the driver's two
+    per-event clears of an unused 501 x 501 u64 image
+    (main/hybridMANTIS_c_ver1_0.c:205-207, 248-250)."""
     import numpy as np
 
     from oracle import pyoracle as po
 
-    kind = "reference" if po.ref_available() else "port"
     cum = np.cumsum(ev["N"].astype(np.int64))
     n_ev = int(np.searchsorted(cum, photons_per_worker * workers)) + 1
     sample = ev[:n_ev]
     photons = int(sample["N"].sum())
-    res, secs = po.run_parallel(p, sample, p["num_hist"], workers=workers,
-                                kind="ref" if kind == "reference" else "oracle")
-    wall = max(secs)
-    out = {"value": photons / wall, "unit": "optical photons/s",
-           "cores": workers, "kind": kind,
-           "sample": f"first {n_ev} events ({photons} photons) of the workload, "
-                     f"{workers} processes over contiguous slices, "
-                     f"slowest worker {wall:.1f} s"}
-    if kind == "reference":
-        # the reference's C driver also clears an unused xdim x ydim u64
-        # image twice per event (main/hybridMANTIS_c_ver1_0.c:205-207,
-        # 248-250); the same run with those clears
-        _, secs_z = po.run_parallel(p, sample, p["num_hist"], workers=workers, kind="ref",
-                                    rezero=True)
-        out["value_with_image_rezero"] = photons / max(secs_z)
-        out["sample"] += f" ({max(secs_z):.1f} s with the driver's per-event image re-zero)"
-    return out
+    po.build()
+    rates = {}
+    for level, path in (("O2", po.LIB_PATH), ("O0", po.LIB_O0_PATH)):
+        for rezero in (False, True):
+            os.environ["FDO_LIBRARY"] = path  # the spawned workers load this build
+            try:
+                _, secs = po.run_parallel(p, sample, p["num_hist"], workers=workers,
+                                          rng=po.RNG_RANECU, seed=123456789,
+                                          math=po.MATH_FAITHFUL, rezero=rezero)
+            finally:
+                os.environ.pop("FDO_LIBRARY", None)
+            rates[(level, rezero)] = (photons / max(secs), max(secs))
+    v, wall = rates[("O2", False)]
+    return {"value": v, "unit": "optical photons/s", "cores": workers, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "value_with_image_rezero": rates[("O2", True)][0],
+            "value_O0": rates[("O0", False)][0],
+            "value_O0_with_image_rezero": rates[("O0", True)][0],
+            "sample": f"first {n_ev} events ({photons} photons) of the workload; the "
+                      "reference C path restated (oracle faithful arithmetic, per-event "
+                      f"RANECU), {workers} processes over contiguous slices; slowest worker "
+                      f"{wall:.1f} s at -O2, {rates[('O0', False)][1]:.1f} s at -O0; value "
+                      "= -O2 without the driver's per-event image clears"}
 
 
 def flops_model(p, ev, n_photons: int = 60000):
@@ -170,7 +202,7 @@ def main() -> int:
     # CPU legs first, before this process touches the GPU
     cpu = None
     if not args.no_cpu:
-        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        workers = args.cpu_workers or host_cores()
         cpu = cpu_baseline(p, ev, workers, args.cpu_photons)
     fpp, per_photon = flops_model(p, ev)
 

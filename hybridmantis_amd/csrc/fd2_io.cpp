@@ -4,6 +4,7 @@
 //   myimage_ / detected_ writers   main/hybridMANTIS_tallyEnergyDepositionEvents.f:
 //                                  418-458
 //   EDE event records               tally .f:173-174 (x y z in cm, de in eV, hist)
+//   console statistics block        tally .f:461-496
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -149,6 +150,41 @@ int fd2_write_phs(const char* path, const int32_t* phs, int32_t num_bins) {
     else
       std::fprintf(f, " %8d\n", phs[i]);
   }
+  return std::fclose(f) == 0 ? FD2_OK : FD2_E_IO;
+}
+
+// Fortran (1x,F12.2): a blank, then the value rounded to 2 decimals right
+// aligned in 12 columns, or 12 asterisks when it does not fit
+static void put_f12_2(FILE* f, double v) {
+  char buf[64];
+  const int len = std::snprintf(buf, sizeof buf, "%12.2f", v);
+  if (len < 0 || len > 12 || !std::isfinite(v))
+    std::fputs(" ************\n", f);
+  else
+    std::fprintf(f, " %s\n", buf);
+}
+
+int fd2_write_stats(const char* path, const double* stats, int rungpu, double penoptical_s) {
+  if (!stats) return FD2_E_INVAL;
+  FILE* f = path ? std::fopen(path, "w") : stdout;
+  if (!f) return FD2_E_IO;
+  static const char* const kArrows = ">>>>>>>>>>>>>>>>>>>>>>>>>>>>>>>>>>>>>>";
+  static const char* const kLabels[6] = {"Generated", "Detected",         "Absorbed at top",
+                                         "Absorbed in bulk", "Lost", "Out of Column"};
+  std::fprintf(f, " \n%s\n    OPTICAL TRANSPORT STATISTICS\n \n", kArrows);
+  std::fputs(rungpu == 1 ? "   !! Simulation in the GPU !!\n" : "   !! Simulation in the CPU !!\n",
+             f);
+  std::fputs(" \nTotal # optical photons:\n", f);
+  for (int k = 0; k < 6; ++k) {  // glgen .. gloutofcol (gltheta1 is not reported)
+    std::fprintf(f, "%s\n", kLabels[k]);
+    put_f12_2(f, stats[k]);
+  }
+  std::fputs("Total GPU or CPU time (s)\n", f);
+  put_f12_2(f, stats[7] * 0.001);  // glgputime is in ms
+  std::fputs("Total optical real time from PENELOPE (s)\n", f);
+  put_f12_2(f, penoptical_s);
+  std::fprintf(f, "%s\n \n", kArrows);
+  if (!path) return std::fflush(f) == 0 ? FD2_OK : FD2_E_IO;
   return std::fclose(f) == 0 ? FD2_OK : FD2_E_IO;
 }
 

@@ -559,6 +559,7 @@ def _io_lib() -> C.CDLL:
             "fd2_write_prf": ([C.c_char_p, C.POINTER(Params), P, C.c_int32, C.c_int32,
                                C.c_double], C.c_int),
             "fd2_write_phs": ([C.c_char_p, P, C.c_int32], C.c_int),
+            "fd2_write_stats": ([C.c_char_p, P, C.c_int, C.c_double], C.c_int),
             "fd2_read_ede": ([C.c_char_p, P, C.c_int64, C.POINTER(C.c_int64)], C.c_int),
         }
         for name, (args, res) in sig.items():
@@ -602,6 +603,17 @@ def write_phs(path: str, phs) -> None:
     phs = np.ascontiguousarray(phs, np.int32)
     _check(_io_lib().fd2_write_phs(os.fsencode(path), phs.ctypes.data, len(phs)),
            f"write {path}")
+
+
+def write_stats(path, stats, rungpu: int = 1, penoptical_s: float = 0.0) -> None:
+    """The reference's console optical-statistics block (fd2_write_stats):
+    stats = the 8 optstats_ doubles; path None writes to standard output."""
+    st = np.ascontiguousarray(stats, np.float64)
+    if st.shape != (8,):
+        raise ValueError("stats: the 8 optstats_ values")
+    _check(_io_lib().fd2_write_stats(None if path is None else os.fsencode(path),
+                                     st.ctypes.data, rungpu, float(penoptical_s)),
+           "write stats")
 
 
 def read_ede(path: str, max_events: int) -> np.ndarray:

@@ -397,6 +397,15 @@ int fd2_write_prf(const char *path, const fd2_params *p, const uint64_t *prf,
                   int32_t xdim, int32_t ydim, double n_hist);
 /* PHS writer (tally .f:456-458): mynumbins lines of (1x,I8). */
 int fd2_write_phs(const char *path, const int32_t *phs, int32_t num_bins);
+/* The console block "OPTICAL TRANSPORT STATISTICS" the reference tally
+ * prints at the end of a run (tally .f:461-496), byte for byte, to `path`
+ * (overwritten) or to standard output when path is NULL.  stats holds the
+ * 8 doubles of COMMON /optstats/ in order (glgen, gldetect, glabstop,
+ * glabsbulk, gllost, gloutofcol, gltheta1, glgputime in ms); rungpu = 1
+ * names the GPU, else the CPU; penoptical_s is the optical wall time the
+ * driver measured (COMMON /tpenoptical/, seconds).  Values print as
+ * (1x,F12.2). */
+int fd2_write_stats(const char *path, const double *stats, int rungpu, double penoptical_s);
 /* EDE text records (3(1x,1pe16.9),1x,1pe12.4,i8): x y z (cm) de (eV)
  * history (tally .f:173-174).  A 72-character line is cut at the record's
  * columns; other lines are read as five blank-separated numbers.  Lines

@@ -1046,6 +1046,21 @@ static void run_event(ctx *c, const fdo_event *ev, int64_t gid, uint32_t seed,
   }
 }
 
+/* The reference C driver's per-event clears of its local xdim x ydim u64
+ * image (drv:205-207 before the event, drv:248-250 after seeding), which the
+ * transport never reads; on only for timing the driver as the reference
+ * runs it.  The image is reachable from this global, so the clears are not
+ * dead stores. */
+static int g_image_rezero;
+unsigned long long *fdo_rezero_image;
+
+void fdo_set_image_rezero(int on) { g_image_rezero = on != 0; }
+
+static void clear_image(unsigned long long *img, int xdim, int ydim) {
+  for (int i = 0; i < xdim; i++)
+    for (int j = 0; j < ydim; j++) img[(int64_t)i * ydim + j] = 0;
+}
+
 int fdo_run(const fdo_geom *g, const fdo_event *events, int64_t n,
             int64_t event_base, int rng_mode, int math_mode, uint32_t seed,
             fdo_tally *t) {
@@ -1062,9 +1077,20 @@ int fdo_run(const fdo_geom *g, const fdo_event *events, int64_t n,
   c.dev = math_mode == FDO_MATH_DEVICE;
   c.emu = math_mode == FDO_MATH_EMULATED;
   fdo_photon *rec = t->photons;
+  if (g_image_rezero) {
+    fdo_rezero_image = (unsigned long long *)malloc((size_t)g->xdim * (size_t)g->ydim *
+                                                   sizeof *fdo_rezero_image);
+    if (!fdo_rezero_image) return -2;
+  }
   for (int64_t i = 0; i < n; ++i) {
+    if (g_image_rezero) clear_image(fdo_rezero_image, g->xdim, g->ydim);
+    if (g_image_rezero) clear_image(fdo_rezero_image, g->xdim, g->ydim);
     run_event(&c, &events[i], event_base + i, seed, rec);
     if (rec) rec += events[i].N > 0 ? events[i].N : 0;
+  }
+  if (g_image_rezero) {
+    free(fdo_rezero_image);
+    fdo_rezero_image = NULL;
   }
   return 0;
 }

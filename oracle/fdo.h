@@ -130,6 +130,11 @@ int fdo_run(const fdo_geom *g, const fdo_event *events, int64_t n,
             int64_t event_base, int rng_mode, int math_mode, uint32_t seed,
             fdo_tally *t);
 
+/* Timing only: repeat the reference C driver's two per-event clears of an
+ * unused xdim x ydim u64 image (main/hybridMANTIS_c_ver1_0.c:205-207,
+ * 248-250) in fdo_run (process-wide setting, off by default). */
+void fdo_set_image_rezero(int on);
+
 /* Restated RANECU pieces, exported for known-answer tests. */
 float fdo_ranecu(int32_t seed[2]);
 void fdo_init_prng(int history_batch, int histories_per_thread, int seed_input,

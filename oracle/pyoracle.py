@@ -14,7 +14,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# FDO_LIBRARY selects another build of the oracle (liboracle_O0.so, the
+# unoptimised build used for timing); spawned workers inherit it
+LIB_PATH = os.environ.get("FDO_LIBRARY") or os.path.join(HERE, "liboracle.so")
+LIB_O0_PATH = os.path.join(HERE, "liboracle_O0.so")
 REF_PATH = os.path.join(HERE, "_ref", "libfdref.so")
 
 RNG_RANECU = 0
@@ -58,7 +61,7 @@ class Tally(C.Structure):
 def build(ref: bool = False) -> None:
     """Compile the oracle (and, if asked and the reference tree exists, the
     reference-branch harness) with oracle/Makefile."""
-    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so", "liboracle_O0.so"], check=True)
     if ref and os.path.exists("/root/reference/main/kernel_cuda_c_ver1_0.cu"):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 
@@ -162,8 +165,9 @@ class Result:
 
 def run(p: dict, events, num_hist: int, rng: int = RNG_COUNTER,
         seed: int = 271828182, event_base: int = 0,
-        record: bool = False, math: int = MATH_FAITHFUL) -> Result:
-    """Transport `events` (EVENT_DTYPE) through the oracle."""
+        record: bool = False, math: int = MATH_FAITHFUL, rezero: bool = False) -> Result:
+    """Transport `events` (EVENT_DTYPE) through the oracle.  rezero repeats
+    the reference C driver's two per-event image clears (timing only)."""
     g = make_geom(p)
     ev = _as_events(events)
     prf = np.zeros((g.xdim, g.ydim), np.uint64)
@@ -176,8 +180,10 @@ def run(p: dict, events, num_hist: int, rng: int = RNG_COUNTER,
     if record:
         photons = np.zeros(int(np.maximum(ev["N"], 0).sum()), PHOTON_DTYPE)
         t.photons = photons.ctypes.data
+    lib().fdo_set_image_rezero(int(rezero))
     rc = lib().fdo_run(C.byref(g), ev.ctypes.data, len(ev), event_base, rng,
                        math, seed & 0xFFFFFFFF, C.byref(t))
+    lib().fdo_set_image_rezero(0)
     if rc != 0:
         raise RuntimeError(f"fdo_run failed ({rc})")
     return Result(prf, hc[:num_hist], np.array(t.counters[:], np.uint64),
@@ -317,7 +323,7 @@ def _slice_worker(args):
         work = None
     else:
         r = run(p, ev, num_hist, rng=rng, seed=seed, event_base=base, math=math,
-                record=record)
+                record=record, rezero=rezero)
         work = r.work
     return (r.prf, r.hist_count, r.counters, work, _t.perf_counter() - t0, r.photons)
 
@@ -325,7 +331,8 @@ def _slice_worker(args):
 def run_parallel(p: dict, events, num_hist: int, workers: int = 8,
                  kind: str = "oracle", rng: int = RNG_COUNTER,
                  seed: int = 271828182, math: int = MATH_FAITHFUL,
-                 rezero: bool = False, timeout: float = 600.0, record: bool = False):
+                 rezero: bool = False, timeout: float = 600.0, record: bool = False,
+                 event_base: int = 0):
     """Run contiguous event slices in separate processes (the transport
     keeps per-process state, like the reference C path) and sum the
     tallies; results equal a single-process run because streams are keyed
@@ -338,7 +345,7 @@ def run_parallel(p: dict, events, num_hist: int, workers: int = 8,
     workers = max(1, min(workers, n))
     bounds = [n * i // workers for i in range(workers + 1)]
     jobs = [(kind, p, ev[bounds[i]:bounds[i + 1]].copy(), num_hist, rng, seed,
-             bounds[i], math, rezero, record) for i in range(workers)]
+             event_base + bounds[i], math, rezero, record) for i in range(workers)]
     # fresh interpreters (spawn): the caller may hold a GPU context; a dead
     # worker raises BrokenProcessPool instead of hanging
     with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:

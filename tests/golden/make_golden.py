@@ -62,6 +62,14 @@ def cases():
     p6 = E.params(num_hist=6, H=500.0, R=5.0, max_ph=5000, num_bins=1000)
     ev6 = E.generate(6, 500.0, 50.0, float(E.attenuation_length_um(50.0)), 15)
     out.append(("thick_50kev", p6, ev6, SEED))
+    # the batched workload's two geometries no other case has: (H, R) =
+    # (100, 2.55) and (300, 4.0) um, polychromatic 15-90 keV
+    for name, H, R, nh, sd in (("thin100_poly", 100.0, 2.55, 30, 16),
+                               ("mid300_r4_poly", 300.0, 4.0, 10, 17)):
+        p7 = E.params(num_hist=nh, H=H, R=R, max_ph=5000, num_bins=1000)
+        e7 = np.random.default_rng(sd).integers(15, 91, nh).astype(np.float64)
+        ev7 = E.generate(nh, H, e7, E.attenuation_length_um(e7), sd)
+        out.append((name, p7, ev7, SEED))
     return out
 
 

@@ -114,3 +114,65 @@ def test_autotuner(fd2lib, batch):
             tr.sync()
         assert sum(tr.counters(k)["generated"] for k in range(len(plist))) == int(ev["N"].sum())
         buf.free()
+
+
+def test_batched_launch_matches_oracle_per_photon(fd2lib, oracle):
+    """A slice of the config-2 workload (250 histories per geometry, all
+    four geometries, events above 2,000 photons) in one multi-geometry
+    launch: each geometry's photons, fates and pixels, against the oracle's
+    device arithmetic on the production streams with that geometry's event
+    numbers in the batch, and each geometry's tallies bit for bit (the
+    per-event target is main/kernel_cuda_c_ver1_0.cu:553-655)."""
+    import os
+
+    plist, ev = E.batched_workload(scale=0.005)
+    assert sorted(np.unique(ev["geom"]).tolist()) == [0, 1, 2, 3]
+    assert ev["N"].max() > 2000
+    print(f"[batched-oracle] {len(ev)} events {int(ev['N'].sum())} photons", flush=True)
+    with fd2lib.Transport(plist) as tr:
+        rec = tr.submit_record(ev)
+        got = tallies(tr, len(plist))
+    off = np.concatenate([[0], np.cumsum(ev["N"].astype(np.int64))])
+    for g, p in enumerate(plist):
+        idx = np.flatnonzero(ev["geom"] == g)
+        lo, hi = int(idx[0]), int(idx[-1]) + 1
+        assert hi - lo == len(idx)  # each geometry's events are contiguous
+        o, _ = oracle.run_parallel(p, ev[lo:hi], p["num_hist"],
+                                   workers=min(16, os.cpu_count() or 1),
+                                   rng=oracle.RNG_COUNTER, seed=271828182,
+                                   math=oracle.MATH_DEVICE, record=True, event_base=lo)
+        r = rec[off[lo]:off[hi]]
+        assert np.array_equal(r["fate"], o.photons["fate"]), g
+        assert np.array_equal(r["pix"], o.photons["pix"]), g
+        prf, hist, cnt = got[g]
+        assert np.array_equal(prf, o.prf), g
+        assert np.array_equal(hist, o.hist_count), g
+        assert [cnt[k] for k in oracle.COUNTER_NAMES] == o.counters.tolist(), g
+
+
+def test_full_batched_workload_properties(fd2lib):
+    """The full config-2 workload (4 x 50,000 histories, ~4.6e8 photons):
+    per geometry every photon ends exactly once, the per-history counts add
+    up to the detections, the PRF holds at most the detections; a graph
+    replay equals a direct submit and an identical rerun."""
+    plist, ev = E.batched_workload()
+    print(f"[batched-full] {len(ev)} events {int(ev['N'].sum())} photons", flush=True)
+    with fd2lib.Transport(plist) as a:
+        buf = a.upload_events(ev)
+        with a.graph(buf, len(ev)) as gr:
+            gr.launch()
+            a.sync()
+        first = tallies(a, len(plist))
+        buf.free()
+    for g, (prf, hist, c) in enumerate(first):
+        n_ph = int(ev["N"][ev["geom"] == g].sum())
+        assert c["generated"] == n_ph
+        ends = sum(c[k] for k in ("detected", "abs_top", "abs_bulk", "lost", "outofcol",
+                                  "theta1"))
+        assert ends == n_ph, g
+        assert int(hist.sum(dtype=np.uint64)) == c["detected"], g
+        assert int(prf.sum()) <= c["detected"], g
+    with fd2lib.Transport(plist) as b:
+        b.submit(ev)
+        b.sync()
+        same(tallies(b, len(plist)), first)

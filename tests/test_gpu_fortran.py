@@ -3,8 +3,9 @@
 The tally's call sequences (main/hybridMANTIS_tallyEnergyDepositionEvents.f:
 198-304, main/hybridMANTIS_penEasy.f:434-443) are driven through the COMMON
 blocks, and the accumulated outputimage_/outputdetprim_/optstats_ must equal
-what the native API gives for the same events, photon counts and seeds
-(bit-exact: PRF, PHS and counters are integers).  The PHS of gpuoptical_
+what the native API gives and what the oracle (its device arithmetic on the
+production streams, the reference's PHS rule) gives for the same events,
+photon counts and seeds (bit-exact: PRF, PHS and counters are integers).  The PHS of gpuoptical_
 covers the whole run; that of cpuoptical_ only the events of its call, as in
 the reference's drivers (hybridMANTIS_cuda_ver1_0.cu:376-399 after call 101,
 hybridMANTIS_c_ver1_0.c:343-366 per call).
@@ -61,12 +62,33 @@ def native(p, buffers, seed):
         return tr.prf(0), tr.phs(0), tr.counters(0)
 
 
+def oracle_tallies(oracle, p, buffers, seed):
+    """The oracle on the same converted events, numbered on across the
+    buffers as one context numbers them: PRF, the reference's PHS rule over
+    the per-history counts (hybridMANTIS_cuda_ver1_0.cu:376-399) and the
+    counters."""
+    ev = np.concatenate(buffers)
+    o = oracle.run(p, ev, p["num_hist"], rng=oracle.RNG_COUNTER, seed=seed,
+                   math=oracle.MATH_DEVICE)
+    phs = oracle.phs(o.hist_count, p["min_ph"], p["max_ph"], p["num_bins"])
+    return o.prf, phs, dict(zip(oracle.COUNTER_NAMES, (int(v) for v in o.counters)))
+
+
+def check_against_oracle(C, want_prf, want_phs, want_cnt):
+    assert np.array_equal(image_part(C, want_prf), want_prf)
+    assert C.image.sum() == want_prf.sum()
+    assert np.array_equal(C.phs[:len(want_phs)], want_phs)
+    st = C.stats()
+    for a, b in COUNTERS:
+        assert st[a] == want_cnt[b], a
+
+
 def image_part(C, prf):
     xd, yd = prf.shape
     return C.image[:xd, :yd]
 
 
-def test_gpuoptical_protocol(shim):
+def test_gpuoptical_protocol(shim, oracle):
     C = shim
     p, ev = E.workload("mono25", scale=0.25)     # 500 histories
     p = shim_params(p)
@@ -88,6 +110,8 @@ def test_gpuoptical_protocol(shim):
     for a, b in COUNTERS:
         assert st[a] == cnt[b], a
     assert st["glgputime"] > 0
+    # and the oracle, bit for bit, after the 99/100/100/101 sequence
+    check_against_oracle(C, *oracle_tallies(oracle, p, conv, SEED))
     # a second run accumulates on top (COMMON blocks are only ever added to)
     F.gpuoptical(99, len(bufs[0]))
     C.fill(ev)
@@ -98,7 +122,7 @@ def test_gpuoptical_protocol(shim):
     assert C.stats()["glgen"] == cnt["generated"] + cnt2["generated"]
 
 
-def test_cpuoptical_per_call_phs_and_range(shim):
+def test_cpuoptical_per_call_phs_and_range(shim, oracle):
     """cpuoptical_(gpusize) runs events [gpusize, myctropt); its PHS covers
     that call's events only."""
     C = shim
@@ -122,6 +146,11 @@ def test_cpuoptical_per_call_phs_and_range(shim):
     st = C.stats()
     for a_, b_ in COUNTERS:
         assert st[a_] == cnt_a[b_] + cnt_b[b_], a_
+    # the oracle: each call's events on that call's seed, each call's own
+    # PHS (hybridMANTIS_c_ver1_0.c:343-366) added into outputdetprim_
+    oa, ob = oracle_tallies(oracle, p, [a], SEED + 1), oracle_tallies(oracle, p, [b], SEED + 2)
+    check_against_oracle(C, oa[0] + ob[0], oa[1] + ob[1],
+                         {k: oa[2][k] + ob[2][k] for k in oa[2]})
 
 
 def test_load_balancing_probe(shim):

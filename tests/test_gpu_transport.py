@@ -36,7 +36,8 @@ from hybridmantis_amd import stats
 pytestmark = pytest.mark.gpu
 
 CASES = ["mono25_small", "example_input", "smooth_nobulk_notop",
-         "low_index_sensor_refl", "detector_edges", "thick_50kev"]
+         "low_index_sensor_refl", "detector_edges", "thick_50kev",
+         "thin100_poly", "mid300_r4_poly"]
 
 
 def load_case(golden, name):

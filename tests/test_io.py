@@ -1,7 +1,8 @@
 """The reference's file formats (fd2_io.cpp), checked against the example
 files the reference ships (example/hybridMANTIS_input.in and
 example/output_files/*.dat, copied under tests/golden/io): the input
-parser, the output-file names, and byte-identical PRF and PHS writers.  The
+parser, the output-file names, byte-identical PRF and PHS writers and the
+console statistics block (1e5xrays_console.out:275-299).  The
 EDE reader is checked on records written in the tally's format
 (main/hybridMANTIS_tallyEnergyDepositionEvents.f:173-174)."""
 import os
@@ -144,3 +145,23 @@ def test_ede_reader(tmp_path):
     bad.write_text("0.01 0.02 x 1500.0 7\n")
     with pytest.raises(fd2.Fd2Error):
         fd2.read_ede(str(bad), 4)
+
+
+def test_console_stats_block_reproduces_example(tmp_path):
+    """fd2_write_stats against the reference example run's console block
+    (example/output_files/1e5xrays_console.out:275-299, copied as
+    tests/golden/io/console_stats_1e5xrays.txt): the counters of that run,
+    its 0.21 s of GPU time and 15.39 s of optical wall time."""
+    from hybridmantis_amd import fd2
+
+    want = open(os.path.join(HERE, "console_stats_1e5xrays.txt"), "rb").read()
+    stats = [95222286.0, 51718347.0, 8184844.0, 3175019.0, 20111907.0, 29389.0, 0.0, 210.0]
+    out = tmp_path / "stats.txt"
+    fd2.write_stats(str(out), stats, rungpu=1, penoptical_s=15.39)
+    assert out.read_bytes() == want
+    fd2.write_stats(str(out), stats, rungpu=0, penoptical_s=15.39)
+    got = out.read_bytes().decode()
+    assert "   !! Simulation in the CPU !!\n" in got
+    # F12.2 overflow prints asterisks, as gfortran does
+    fd2.write_stats(str(out), [1e13] + stats[1:], rungpu=1, penoptical_s=0.0)
+    assert out.read_text().splitlines()[8] == " ************"

@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 CASES = ["mono25_small", "example_input", "smooth_nobulk_notop",
-         "low_index_sensor_refl", "detector_edges", "thick_50kev"]
+         "low_index_sensor_refl", "detector_edges", "thick_50kev",
+         "thin100_poly", "mid300_r4_poly"]
 
 
 def load(golden, name):
