@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -66,20 +67,19 @@ def build(verbose: bool = False, jobs: int = 4, out: str | None = None,
                    "-o", obj]
         cmds.append(cmd)
         objs.append(obj)
-    procs = []
-    for cmd in cmds:
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
-        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE,
-                                      stderr=subprocess.STDOUT, text=True))
-        while len([p for p in procs if p.poll() is None]) >= jobs:
-            procs[0].wait()
+        # each job's output is collected whole, so no pipe can fill and stall it
+        return subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+    with ThreadPoolExecutor(max(1, jobs)) as pool:
+        results = list(pool.map(compile_one, cmds))
     failed = []
-    for cmd, p in zip(cmds, procs):
-        log, _ = p.communicate()
-        if log and (verbose or p.returncode):
-            sys.stderr.write(log)
-        if p.returncode:
+    for cmd, r in zip(cmds, results):
+        if r.stdout and (verbose or r.returncode):
+            sys.stderr.write(r.stdout)
+        if r.returncode:
             failed.append(cmd[-3])
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")

@@ -421,6 +421,8 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { rc = FD2_E_DEVICE; break; }
     c->n_cu = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    c->lds_static = transport_static_lds();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = FD2_E_DEVICE; break; }
     if (hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
       rc = FD2_E_DEVICE;
@@ -514,6 +516,12 @@ int fd2_set_tuning(fd2_ctx* c, const fd2_tuning* t) {
     n.waves_per_cu = t->waves_per_cu;
   }
   if (t->lds_tile < -1 || t->lds_tile > kTileMax) return FD2_E_INVAL;
+  {  // the tiles and the kernel's static LDS must fit one workgroup's LDS
+    int32_t side = 0, n_tiles = 0;
+    tile_shape(t->lds_tile, c->n_geoms, &side, &n_tiles);
+    const size_t tiles = (size_t)n_tiles * side * side * sizeof(uint32_t);
+    if (c->lds_per_block && tiles + c->lds_static > c->lds_per_block) return FD2_E_INVAL;
+  }
   n.lds_tile = t->lds_tile;
   if (t->park_lanes) {
     if (t->park_lanes < 1 || t->park_lanes > 64) return FD2_E_INVAL;

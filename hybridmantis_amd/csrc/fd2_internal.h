@@ -54,7 +54,10 @@ static_assert(kQSlots % kQBlock == 0 && kWallCap % kQBlock == 0 && kGapBase % kQ
                   kBackBase % kQBlock == 0,
               "queue regions are whole blocks");
 constexpr uint32_t kMaxIdlePasses = 256;  // passes without a step or a wall branch
-constexpr int kTileMax = 112;  // 49 KiB + 8 KiB static: within 64 KiB per workgroup
+// largest LDS tile side: 49 KiB of tile, plus the kernel's static LDS (about
+// 16 KiB with the several-geometry table); fd2_set_tuning checks the sum
+// against the device's LDS per workgroup (160 KiB on gfx950)
+constexpr int kTileMax = 112;
 
 struct QueueArgs {
   const fd2_event* events;
@@ -97,6 +100,7 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 hipError_t launch_set_u64(unsigned long long* p, unsigned long long v, hipStream_t s);
 int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem, bool queue);
+size_t transport_static_lds();  // largest static LDS of the transport kernels
 
 }  // namespace fd2
 
@@ -104,6 +108,8 @@ int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t sme
 struct fd2_ctx {
   int device = 0;
   int n_cu = 0;
+  size_t lds_per_block = 0;   // the device's LDS per workgroup
+  size_t lds_static = 0;      // the transport kernels' static LDS (largest)
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   bool timed = false;

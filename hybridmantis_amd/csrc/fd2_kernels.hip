@@ -796,6 +796,26 @@ static int occupancy_of(size_t smem, bool queue) {
   return nb > 0 ? nb : 1;
 }
 
+template <bool MULTI, bool REC, bool RADIAL, bool QUEUE>
+static size_t static_lds_of() {
+  hipFuncAttributes at{};
+  if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(
+                                    &k_transport<MULTI, REC, RADIAL, QUEUE>)) != hipSuccess)
+    return 0;
+  return at.sharedSizeBytes;
+}
+
+size_t transport_static_lds() {
+  // the several-geometry variants hold the geometry table: the largest
+  const size_t a = static_lds_of<true, false, false, true>();
+  const size_t b = static_lds_of<true, true, false, true>();
+  const size_t c = static_lds_of<true, false, true, true>();
+  const size_t d = static_lds_of<true, false, false, false>();
+  size_t m = a > b ? a : b;
+  m = m > c ? m : c;
+  return m > d ? m : d;
+}
+
 int transport_max_blocks_per_cu(bool multi, bool record, bool radial, size_t smem, bool queue) {
   if (radial)
     return multi ? occupancy_of<true, false, true>(smem, queue)

@@ -725,7 +725,8 @@ __global__ void k_surr_gather(const fd2_event* __restrict__ all, const uint32_t*
   ids[i] = base + k;
 }
 
-// edges of the radial bins: 0, 0.5 * 400^((j-1)/62) for j = 1..63, 300 um
+// edges of the radial bins: 0, then 0.5 * 1600^((j-1)/62) um for j = 1..63
+// (0.5 to 800 um), then 1000 um: bin 0 = [0, 0.5), 62 log bins, [800, 1000)
 __device__ __forceinline__ float radial_edge(int j) {
   if (j <= 0) return 0.0f;
   if (j >= FD2_RADIAL_BINS) return 1000.0f;
