@@ -36,22 +36,23 @@ struct TransportArgs {
 // LDS PRF tiles of the persistent kernel: geometry g < n_tiles counts the
 // detections in a tile_side^2 pixel square centred on its PRF image in LDS
 // (u32) and adds the block's counts to the PRF at the end.
-constexpr int kTileDefault = 64;
+constexpr int kTileDefault = 0;  // no tile: the LDS holds the wall queues
 constexpr int kTileBudget = 64 * 64;   // u32 entries per block (16 KiB)
 // started photons of a wave's chunk in LDS (direction + stream words)
 constexpr int kStartWords = 7;
-// per-wave photon queues of the persistent kernel: wall queue, gap queue,
-// backlog (slots in that order)
-constexpr int kWallCap = 128;
+// per-wave photon queues of the persistent kernel: the wall queue in LDS
+// (kWallCap slots of kWallWords words, word-major), the gap queue and the
+// backlog in HBM (slots in that order)
+constexpr int kWallCap = 64;
+constexpr int kWallWords = 16;  // the photon words of a queue entry
 constexpr int kGapCap = 128;
 constexpr int kBackCap = 64 + kWallCap + kGapCap;
-constexpr int kGapBase = kWallCap;
-constexpr int kBackBase = kWallCap + kGapCap;
-constexpr int kQSlots = kWallCap + kGapCap + kBackCap;
+constexpr int kGapBase = 0;
+constexpr int kBackBase = kGapCap;
+constexpr int kQSlots = kGapCap + kBackCap;
 constexpr int kQWords = 20;  // 16 photon words + 4 of wall-branch state
 constexpr int kQBlock = 32;  // queue slots per block (word stride 128 bytes)
-static_assert(kQSlots % kQBlock == 0 && kWallCap % kQBlock == 0 && kGapBase % kQBlock == 0 &&
-                  kBackBase % kQBlock == 0,
+static_assert(kQSlots % kQBlock == 0 && kGapBase % kQBlock == 0 && kBackBase % kQBlock == 0,
               "queue regions are whole blocks");
 constexpr uint32_t kMaxIdlePasses = 256;  // passes without a step or a wall branch
 // largest LDS tile side: 49 KiB of tile, plus the kernel's static LDS (about

@@ -300,33 +300,41 @@ __device__ __forceinline__ size_t q_base(uint32_t slot) {
   return (size_t)(slot / kQBlock) * (kQBlock * kQWords) + slot % kQBlock;
 }
 
-// The geometry index is stored only for several geometries per launch
-// (MULTI; else 0) and the photon index only for per-photon records (REC).
+// An entry's words lie `S` words apart from its first: kQBlock in the HBM
+// queues, kWallCap in a wave's LDS wall queue (word-major, one row of
+// kWallCap slots per word).  The geometry index is stored only for several
+// geometries per launch (MULTI; else 0) and the photon index only for
+// per-photon records (REC).
+template <bool MULTI, bool REC, int S = kQBlock>
+__device__ __forceinline__ void q_put_at(uint32_t* w, const Photon& p, const CounterRng& r,
+                                         uint32_t ev, uint32_t gi, uint32_t k, bool parked) {
+  w[0 * S] = __float_as_uint(p.px);
+  w[1 * S] = __float_as_uint(p.py);
+  w[2 * S] = __float_as_uint(p.pz);
+  w[3 * S] = __float_as_uint(p.ux);
+  w[4 * S] = __float_as_uint(p.uy);
+  w[5 * S] = __float_as_uint(p.uz);
+  w[6 * S] = __float_as_uint(p.xc);
+  w[7 * S] = __float_as_uint(p.yc);
+  // loops (21 bits) | sensor resume (bit 22) | parked | new-column sample
+  // due, with its rejected samples (<= 100) in bits 25-31
+  w[8 * S] = (uint32_t)p.loops |
+             (p.fate == kResume ? 1u << 22 : 0u) | (parked ? 1u << 23 : 0u) |
+             (p.fate == kNewColumn ? (1u << 24) | ((uint32_t)p.pix << 25) : 0u);
+  w[9 * S] = r.s0;
+  w[10 * S] = r.s1;
+  w[11 * S] = r.s2;
+  w[12 * S] = r.s3;
+  w[13 * S] = ev;
+  if (MULTI) w[14 * S] = gi;
+  if (REC) w[15 * S] = k;
+}
+
 template <bool MULTI, bool REC>
 __device__ __forceinline__ void q_put(uint32_t* wq, uint32_t slot, const Photon& p,
                                       const CounterRng& r, uint32_t ev, uint32_t gi, uint32_t k,
                                       bool parked) {
-  uint32_t* w = wq + q_base(slot);
-  w[0 * kQBlock] = __float_as_uint(p.px);
-  w[1 * kQBlock] = __float_as_uint(p.py);
-  w[2 * kQBlock] = __float_as_uint(p.pz);
-  w[3 * kQBlock] = __float_as_uint(p.ux);
-  w[4 * kQBlock] = __float_as_uint(p.uy);
-  w[5 * kQBlock] = __float_as_uint(p.uz);
-  w[6 * kQBlock] = __float_as_uint(p.xc);
-  w[7 * kQBlock] = __float_as_uint(p.yc);
-  // loops (21 bits) | sensor resume (bit 22) | parked | new-column sample
-  // due, with its rejected samples (<= 100) in bits 25-31
-  w[8 * kQBlock] = (uint32_t)p.loops |
-                    (p.fate == kResume ? 1u << 22 : 0u) | (parked ? 1u << 23 : 0u) |
-                    (p.fate == kNewColumn ? (1u << 24) | ((uint32_t)p.pix << 25) : 0u);
-  w[9 * kQBlock] = r.s0;
-  w[10 * kQBlock] = r.s1;
-  w[11 * kQBlock] = r.s2;
-  w[12 * kQBlock] = r.s3;
-  w[13 * kQBlock] = ev;
-  if (MULTI) w[14 * kQBlock] = gi;
-  if (REC) w[15 * kQBlock] = k;
+  q_put_at<MULTI, REC>(wq + q_base(slot), p, r, ev, gi, k, parked);
 }
 
 __device__ __forceinline__ void q_put_state(uint32_t* wq, uint32_t slot, const PrptState& st) {
@@ -345,31 +353,36 @@ __device__ __forceinline__ void q_get_state(const uint32_t* wq, uint32_t slot, P
   st.c2 = __uint_as_float(w[19 * kQBlock]);
 }
 
-template <bool MULTI, bool REC>
-__device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon& p, CounterRng& r,
-                                      uint32_t& ev, uint32_t& gi, uint32_t& k, bool& parked) {
-  const uint32_t* w = wq + q_base(slot);
-  p.px = __uint_as_float(w[0 * kQBlock]);
-  p.py = __uint_as_float(w[1 * kQBlock]);
-  p.pz = __uint_as_float(w[2 * kQBlock]);
-  p.ux = __uint_as_float(w[3 * kQBlock]);
-  p.uy = __uint_as_float(w[4 * kQBlock]);
-  p.uz = __uint_as_float(w[5 * kQBlock]);
+template <bool MULTI, bool REC, int S = kQBlock>
+__device__ __forceinline__ void q_get_at(const uint32_t* w, Photon& p, CounterRng& r,
+                                         uint32_t& ev, uint32_t& gi, uint32_t& k, bool& parked) {
+  p.px = __uint_as_float(w[0 * S]);
+  p.py = __uint_as_float(w[1 * S]);
+  p.pz = __uint_as_float(w[2 * S]);
+  p.ux = __uint_as_float(w[3 * S]);
+  p.uy = __uint_as_float(w[4 * S]);
+  p.uz = __uint_as_float(w[5 * S]);
   p.nx = p.ny = p.nz = 0.0f;  // written before it is read
-  p.xc = __uint_as_float(w[6 * kQBlock]);
-  p.yc = __uint_as_float(w[7 * kQBlock]);
-  const uint32_t f = w[8 * kQBlock];
+  p.xc = __uint_as_float(w[6 * S]);
+  p.yc = __uint_as_float(w[7 * S]);
+  const uint32_t f = w[8 * S];
   p.loops = (int32_t)(f & 0x1fffffu);
   p.fate = ((f >> 22) & 1u) ? kResume : ((f >> 24) & 1u) ? kNewColumn : kNone;
   parked = (f >> 23) & 1u;
   p.pix = ((f >> 24) & 1u) ? (int32_t)(f >> 25) : -1;
-  r.s0 = w[9 * kQBlock];
-  r.s1 = w[10 * kQBlock];
-  r.s2 = w[11 * kQBlock];
-  r.s3 = w[12 * kQBlock];
-  ev = w[13 * kQBlock];
-  gi = MULTI ? w[14 * kQBlock] : 0u;
-  k = REC ? w[15 * kQBlock] : 0u;
+  r.s0 = w[9 * S];
+  r.s1 = w[10 * S];
+  r.s2 = w[11 * S];
+  r.s3 = w[12 * S];
+  ev = w[13 * S];
+  gi = MULTI ? w[14 * S] : 0u;
+  k = REC ? w[15 * S] : 0u;
+}
+
+template <bool MULTI, bool REC>
+__device__ __forceinline__ void q_get(const uint32_t* wq, uint32_t slot, Photon& p, CounterRng& r,
+                                      uint32_t& ev, uint32_t& gi, uint32_t& k, bool& parked) {
+  q_get_at<MULTI, REC>(wq + q_base(slot), p, r, ev, gi, k, parked);
 }
 
 // number of set bits of a wave-wide mask below this lane (v_mbcnt)
@@ -440,6 +453,8 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
   // photon start_base + j has its first direction in words 0-2 of slot j
   // and its random stream (after the initial draws) in words 3-6
   __shared__ uint32_t s_start[kTransportThreads / 64][kStartWords][64];
+  // the waves' wall queues (QUEUE): entries in the layout of q_put_at, word-major
+  __shared__ uint32_t s_wall[QUEUE ? kTransportThreads / 64 : 1][kWallWords][kWallCap];
   extern __shared__ uint32_t s_tile[];  // n_tiles x tile_side^2
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x) s_cnt[i] = 0;
   const int side = a.tile_side;
@@ -476,8 +491,10 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
   uint32_t lane_k = 0;   // photon index within the event
   StepSink nosink{nullptr, nullptr, 0, 0, false};
 
-  // wave queues (QUEUE): wall [0, n_wall), gap [0, n_gap), backlog [0, n_back)
+  // wave queues (QUEUE): wall [0, n_wall) in LDS, gap [0, n_gap) and backlog
+  // [0, n_back) in HBM
   uint32_t* wq = nullptr;
+  uint32_t* wall = &s_wall[QUEUE ? threadIdx.x >> 6 : 0][0][0];
   uint32_t n_wall = 0, n_gap = 0, n_back = 0, idle = 0;
   if (QUEUE)
     wq = a.wave_queue + (size_t)(blockIdx.x * (kTransportThreads / 64) + (threadIdx.x >> 6)) *
@@ -613,7 +630,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
         const uint32_t k = min((uint32_t)__popcll(pm), (uint32_t)kWallCap - n_wall);
         const uint32_t rank = rank_below(pm);
         if (alive && parked && rank < k) {
-          q_put<MULTI, REC>(wq, n_wall + rank, p, rng, lane_ev, gi, lane_k, false);
+          q_put_at<MULTI, REC, kWallCap>(wall + n_wall + rank, p, rng, lane_ev, gi, lane_k, false);
           alive = false;
           parked = false;
         }
@@ -634,10 +651,14 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
           break;
         }
         const uint32_t take = min(n_wall, 64u);
-        __threadfence_block();
+        // the wall entries are this wave's LDS writes, which complete in
+        // order; the barrier keeps the compiler from moving the reads
+        // above them
+        __builtin_amdgcn_wave_barrier();
         if ((uint32_t)lane < take) {
           bool pk;
-          q_get<MULTI, REC>(wq, n_wall - take + (uint32_t)lane, p, rng, lane_ev, gi, lane_k, pk);
+          q_get_at<MULTI, REC, kWallCap>(wall + n_wall - take + (uint32_t)lane, p, rng, lane_ev,
+                                         gi, lane_k, pk);
           alive = true;
         }
         n_wall -= take;
