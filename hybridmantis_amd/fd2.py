@@ -407,6 +407,7 @@ def _mlp_lib() -> C.CDLL:
             "fd2_mlp_set_data": ([P, P, P, C.c_int64], C.c_int),
             "fd2_mlp_train": ([P, C.c_int, C.c_int, C.c_float, C.c_uint32, P,
                                C.POINTER(C.c_float)], C.c_int),
+            "fd2_mlp_time_forward": ([P, C.c_int, C.c_int, P], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -478,6 +479,14 @@ class Mlp:
         _check(_mlp_lib().fd2_mlp_set_data(self._h, self._data_x.ctypes.data,
                                            self._data_y.ctypes.data, len(self._data_x)),
                "mlp_set_data")
+
+    def time_forward(self, batch: int = 65536, reps: int = 20) -> float:
+        """Mean GPU ms of the forward GEMMs on a device-resident batch
+        (fd2_mlp_time_forward)."""
+        ms = C.c_float()
+        _check(_mlp_lib().fd2_mlp_time_forward(self._h, batch, reps, C.byref(ms)),
+               "mlp_time_forward")
+        return float(ms.value)
 
     def train(self, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0):
         losses = np.zeros(steps, np.float32)

@@ -11,10 +11,17 @@ from the network.  Both start from the events in host memory and are timed
 to the tallies being final (wall clock, so routing, copies and launches are
 included).  Reported: histories/s of both and their ratio, the normalised
 PRF L1 distance, the detected-fraction error, and mean detected photons and
-Swank factor of both PHS.  The PRF L1 distance is given at the workload's
-1-um pitch and at 9 um (the reference example's pitch).  With --noise-floor
-a second plain-MC run with another seed gives the L1 distances two MC runs
-of this size have anyway.
+Swank factor of both PHS.
+
+PRF error.  At the workload's 1-um pitch (501 x 501 pixels) two plain-MC
+runs of the same 1e6 histories differ by about 0.02 in L1 from photon
+statistics alone, so an L1 against one MC run cannot resolve a 2% bar.  The
+surrogate's PRF is therefore scored against a higher-statistics reference:
+the mean PRF of `ref_runs` plain-MC runs of the same events with other
+seeds; the timed plain-MC run is scored against the same reference, which
+gives the L1 an unbiased method of this size has.  The L1 against the timed
+MC run itself and the 9-um (the reference example's pitch) L1 are reported
+too.  With --noise-floor a second plain-MC run gives the MC-vs-MC L1.
 """
 from __future__ import annotations
 
@@ -48,8 +55,11 @@ def _surrogate(p, ev, net, seed, cfg):
         return dt, tr.prf(0), tr.hist_counts(0), tr.counters(0), st
 
 
+PRF_L1_BAR = 0.02
+
+
 def evaluate(scale: float = 1.0, net=None, seed: int = 271828182, noise_floor: bool = False,
-             log=None, **cfg) -> dict:
+             log=None, ref_runs: int = 4, **cfg) -> dict:
     """Run both paths on workload mono50 at `scale` and compare them."""
     from . import surrogate as S
 
@@ -90,6 +100,19 @@ def evaluate(scale: float = 1.0, net=None, seed: int = 271828182, noise_floor: b
             swank_mc=sw_mc, swank_surrogate=sw_s, swank_rel_err=abs(sw_s / sw_mc - 1.0),
             routing={k: (float(v) if isinstance(v, float) else int(v)) for k, v in st.items()},
         )
+        if ref_runs > 0:
+            ref = None
+            for k in range(ref_runs):
+                _, prf_k, _, _, _ = _mc(p, ev, seed + 10 + k)
+                ref = prf_k.astype(np.float64) if ref is None else ref + prf_k
+            out.update(
+                prf_l1_ref_runs=ref_runs,
+                prf_l1_vs_ref=stats.l1_distance(prf_s, ref),
+                prf_l1_mc_vs_ref=stats.l1_distance(prf_mc, ref),
+                prf_l1_9um_vs_ref=stats.l1_distance(coarse(prf_s), coarse(ref)),
+                prf_l1_9um_mc_vs_ref=stats.l1_distance(coarse(prf_mc), coarse(ref)),
+                prf_l1_bar=PRF_L1_BAR)
+            out["prf_l1_pass"] = bool(out["prf_l1_vs_ref"] <= PRF_L1_BAR)
         if noise_floor:
             _, prf_mc2, hist_mc2, c_mc2, _ = _mc(p, ev, seed + 2)
             out["prf_l1_mc_vs_mc"] = stats.l1_distance(prf_mc2, prf_mc)
@@ -108,8 +131,9 @@ def main(argv=None) -> int:
     ap.add_argument("--noise-floor", action="store_true")
     ap.add_argument("--audit", type=float, default=0.01)
     ap.add_argument("--band", type=float, default=5.0)
+    ap.add_argument("--ref-runs", type=int, default=4)
     a = ap.parse_args(argv)
-    rep = evaluate(a.scale, noise_floor=a.noise_floor, audit_fraction=a.audit,
+    rep = evaluate(a.scale, noise_floor=a.noise_floor, ref_runs=a.ref_runs, audit_fraction=a.audit,
                    surface_band=a.band, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(rep), flush=True)
     return 0

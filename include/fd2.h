@@ -302,6 +302,12 @@ int fd2_mlp_gradients(fd2_mlp *mlp, const float *x, const float *y, int batch,
 int fd2_mlp_set_data(fd2_mlp *mlp, const float *x, const float *y, int64_t n);
 int fd2_mlp_train(fd2_mlp *mlp, int steps, int batch, float lr, uint32_t seed,
                   float *losses, float *ms);
+/* Measurement: the network's forward GEMMs (4 layers, BF16 MFMA, FP32
+ * accumulate, bias + GELU epilogues) on a device-resident batch of `batch`
+ * rows (multiple of 16, the inputs currently in the network's batch
+ * buffer), `reps` times; *ms is the mean GPU time of one forward (no host
+ * copies).  2 x 148,992 FLOP per row. */
+int fd2_mlp_time_forward(fd2_mlp *mlp, int batch, int reps, float *ms);
 
 /* Surrogate transport with plain-MC fallback.  Each event goes to the plain
  * Monte Carlo when it lies within surface_band of the top or bottom surface,

@@ -143,23 +143,25 @@ def test_surrogate_pipeline_small(fd2lib):
 
 def test_trained_surrogate_gain(fd2lib):
     """The shipped network (hybridmantis_amd/data/surrogate_mlp.npz) against
-    the method without it: 50,000 histories at 50 keV on a 500-um layer,
-    surrogate + MC fallback vs the plain MC on the same events.  It must be
-    several times faster and agree to the MC's own run-to-run noise: PRF L1
-    within 1.5x the L1 between two plain-MC runs (+0.01), detected fraction
-    and Swank factor within 1%."""
+    the method without it, at the config's own size: 1e6 histories at 50 keV
+    on a 500-um layer, surrogate + MC fallback vs the plain MC on the same
+    events, at north_star's bars: at least 10x the plain MC's histories/s,
+    PRF L1 <= 2% at the 1-um pitch (against the mean of 4 plain-MC runs of
+    the same events, since two single runs already differ by ~2% there) and
+    within 1.25x of what a plain-MC run of the same size scores against that
+    reference, detected fraction within 1%, Swank factor within 1%."""
     from hybridmantis_amd import surrogate_eval
 
-    r = surrogate_eval.evaluate(0.05, noise_floor=True)
-    print({k: r[k] for k in ("speedup", "prf_l1", "prf_l1_mc_vs_mc", "prf_l1_9um",
-                             "prf_l1_9um_mc_vs_mc", "detected_fraction_rel_err",
-                             "swank_rel_err")}, flush=True)
+    r = surrogate_eval.evaluate(1.0, ref_runs=4)
+    print({k: r[k] for k in ("speedup", "prf_l1", "prf_l1_vs_ref", "prf_l1_mc_vs_ref",
+                             "prf_l1_9um_vs_ref", "prf_l1_9um_mc_vs_ref",
+                             "detected_fraction_rel_err", "swank_rel_err")}, flush=True)
     assert r["routing"]["n_surrogate"] > 0.9 * r["events"]
-    assert r["speedup"] > 5.0
-    assert r["detected_fraction_rel_err"] < 0.01
-    assert r["swank_rel_err"] < 0.01
-    assert r["prf_l1"] < 1.5 * r["prf_l1_mc_vs_mc"] + 0.01
-    assert r["prf_l1_9um"] < 1.5 * r["prf_l1_9um_mc_vs_mc"] + 0.01
+    assert r["speedup"] >= 10.0
+    assert r["detected_fraction_rel_err"] <= 0.01
+    assert r["swank_rel_err"] <= 0.01
+    assert r["prf_l1_vs_ref"] <= 0.02
+    assert r["prf_l1_vs_ref"] <= 1.25 * r["prf_l1_mc_vs_ref"]
 
 
 def _tallies(tr):
