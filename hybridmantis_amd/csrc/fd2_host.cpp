@@ -408,7 +408,7 @@ int fd2_create(const fd2_params* params, int n_geoms, int device, fd2_ctx** out)
   c->device = device;
   c->n_geoms = n_geoms;
   c->tune.chunk = 128;
-  c->tune.waves_per_cu = 16;
+  c->tune.waves_per_cu = 20;
   c->tune.lds_tile = 0;
   c->tune.park_lanes = 32;
   c->tune.refill_lanes = 4;
@@ -1011,9 +1011,9 @@ int fd2_autotune(fd2_ctx* c, const fd2_event* d_events, int64_t n, int32_t repea
   // coordinate descent from the context's current shape, two sweeps over
   // chunk, waves per CU, parked lanes and LDS tile side
   static const int chunks[] = {16, 32, 64, 128, 256};
-  static const int waves[] = {8, 12, 16};
+  static const int waves[] = {12, 16, 20};
   static const int parks[] = {8, 16, 32, 48};
-  static const int tiles[] = {-1, 32, 48, kTileDefault, 80};
+  static const int tiles[] = {-1, 32, 48, 64, 80};
   static const int refills[] = {1, 4, 8, 16, 32};
   static const int queues[] = {-1, 1};
   fd2_tuning cur = c->tune;

@@ -197,15 +197,20 @@ __device__ __forceinline__ void counter_initial_dir(CounterRng& rng, Photon& p,
 
 // ------------------------------------------------------- transport
 
-// Register budget of the transport kernel: 4 waves per SIMD (128 VGPRs) runs
-// faster than 3 waves with no spills, although a few values then live in
-// scratch (measured on MI355X at 1e5 histories).  FD2_WAVES_PER_EU=0 lets
-// the compiler choose.
+// Register budget of the transport kernel: 5 waves per SIMD (96 VGPRs).  The
+// wall branch's live set needs 114 VGPRs (66 without it), so 10 values are
+// spilled to scratch at this budget; the fifth wave still wins: 149.8 ->
+// 140.6 ms per mono25x50 launch on MI355X at 20 waves per CU (6 waves, 80
+// VGPRs and 25 spilled values, lose).  FD2_WAVES_PER_EU=0 lets the compiler
+// choose.
 #ifndef FD2_WAVES_PER_EU
-#define FD2_WAVES_PER_EU 4
+#define FD2_WAVES_PER_EU 5
 #endif
+// With several geometries per launch (the geometry table in LDS, read per
+// lane) the kernel needs more registers and keeps 4 waves per SIMD.
 #if FD2_WAVES_PER_EU > 0
-#define FD2_TRANSPORT_ATTR __attribute__((amdgpu_waves_per_eu(FD2_WAVES_PER_EU)))
+#define FD2_TRANSPORT_ATTR \
+  __attribute__((amdgpu_waves_per_eu(MULTI ? 4 : FD2_WAVES_PER_EU, MULTI ? 4 : FD2_WAVES_PER_EU)))
 #else
 #define FD2_TRANSPORT_ATTR
 #endif

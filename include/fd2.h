@@ -106,11 +106,13 @@ enum {
 /* Launch shape of the persistent transport kernel. 0 = library default. */
 typedef struct fd2_tuning {
   int32_t chunk;            /* photons per work-queue item (power of 2)     */
-  int32_t waves_per_cu;     /* resident waves per CU (workgroups of 4 waves) */
+  int32_t waves_per_cu;     /* resident waves per CU (workgroups of 4 waves;
+                             * default 20)                                  */
   int32_t lds_tile;         /* side (pixels, <= 112) of the LDS PRF tiles,
                              * centred on each PRF image, of the first
                              * min(n_geoms, 4096 / side^2) geometries;
-                             * 0 = default (64), -1 = no tiles              */
+                             * 0 = default (none: the LDS holds the waves'
+                             * wall queues), -1 = no tiles                 */
   int32_t park_lanes;       /* lanes of a wave that wait for the Fresnel /
                              * inter-column phase before it runs (1-64)     */
   int32_t refill_lanes;     /* free lanes of a wave before new photons are
@@ -371,7 +373,7 @@ void fd2_graph_destroy(fd2_graph *graph);
 /* Launch-shape autotuner: times the transport of n device events on a
  * scratch context with the same parameters (the context's tallies are not
  * touched) for candidate shapes -- coordinate descent, two sweeps over chunk
- * {16..256}, waves per CU {8, 12, 16}, parked lanes {8..48}, LDS tile side
+ * {16..256}, waves per CU {12, 16, 20}, parked lanes {8..48}, LDS tile side
  * {none, 32..80}, refill lanes {1..32} and wall queue {off, on}, each the
  * best of `repeats` runs after a warm-up --
  * and sets the fastest on ctx.  Candidates and their times go to results
