@@ -478,13 +478,16 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
   const uint64_t event_base = a.event_base_dev ? *a.event_base_dev : a.event_base;
 
   const int lane = threadIdx.x & 63;
+  // the wave's index in the block, uniform across the wave: the compiler
+  // then keeps the wave's queue and LDS bases in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint32_t chunk = 1u << a.chunk_log2;
 
   // wave-uniform work pool: photons [pool_next, pool_end) of event pool_ev;
   // those in [start_base, start_end) (up to 64) are started in s_start
   int64_t pool_ev = 0;
   int32_t pool_next = 0, pool_end = 0, start_base = 0, start_end = 0;
-  uint32_t (*start)[64] = s_start[threadIdx.x >> 6];
+  uint32_t (*start)[64] = s_start[wave];
   bool pool_done = false;
 
   Photon p;
@@ -499,10 +502,10 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
   // wave queues (QUEUE): wall [0, n_wall) in LDS, gap [0, n_gap) and backlog
   // [0, n_back) in HBM
   uint32_t* wq = nullptr;
-  uint32_t* wall = &s_wall[QUEUE ? threadIdx.x >> 6 : 0][0][0];
+  uint32_t* wall = &s_wall[QUEUE ? wave : 0][0][0];
   uint32_t n_wall = 0, n_gap = 0, n_back = 0, idle = 0;
   if (QUEUE)
-    wq = a.wave_queue + (size_t)(blockIdx.x * (kTransportThreads / 64) + (threadIdx.x >> 6)) *
+    wq = a.wave_queue + (size_t)(blockIdx.x * (kTransportThreads / 64) + wave) *
                             (kQWords * kQSlots);
 
   for (;;) {
