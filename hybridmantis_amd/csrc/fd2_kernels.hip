@@ -164,6 +164,19 @@ hipError_t launch_queue_build(const QueueArgs& q, hipStream_t s) {
 
 // ------------------------------------------------------------ tallies
 
+// Tally atomics (PRF, history counts, radial bins, records' slots) through
+// global (address space 1) pointers.  The tally pointers are read from the
+// geometry table, so the compiler cannot tell their address space and would
+// issue flat atomics; a flat operation counts against lgkmcnt as well as
+// vmcnt, and the wave's next wait for an LDS or scalar load would then wait
+// for the atomic to reach L2 (on every detection, at the pencil beam's few
+// hot pixels).  A global atomic without return counts against vmcnt only.
+template <class T>
+__device__ __forceinline__ void tally_add(T* p, T v) {
+  __hip_atomic_fetch_add((__attribute__((address_space(1))) T*)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ int fate_counter(int fate) {
   switch (fate) {
     case kDetect:
@@ -240,7 +253,7 @@ __device__ __forceinline__ void flush_run(const GeomDev* __restrict__ geoms,
   const fd2_event& ev = events[run_ev];
   const int32_t hist = ev.hist;
   const GeomDev& g = geoms[MULTI ? ev.geom : 0u];
-  if (hist >= 1 && hist <= g.num_hist) atomicAdd(&g.hist[hist - 1], run_cnt);
+  if (hist >= 1 && hist <= g.num_hist) tally_add(&g.hist[hist - 1], run_cnt);
 }
 
 // Tallies of an ended photon: fate counter, PRF pixel (LDS tile or HBM),
@@ -267,7 +280,7 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a,
           in_tile = true;
         }
       }
-      if (!in_tile) atomicAdd(&gt.prf[pix_index(p.pix, gt)], 1ull);
+      if (!in_tile) tally_add(&gt.prf[pix_index(p.pix, gt)], 1ull);
     }
     if (lane_ev == run[0]) {
       ++run[1];
@@ -279,7 +292,7 @@ __device__ __forceinline__ void finish_photon(const TransportArgs& a,
   }
   if (RADIAL && det) {
     const fd2_event& ev = events[lane_ev];
-    atomicAdd(&a.radial[(size_t)lane_ev * FD2_RADIAL_BINS + radial_bin(p.px - ev.x, p.py - ev.y)],
+    tally_add(&a.radial[(size_t)lane_ev * FD2_RADIAL_BINS + radial_bin(p.px - ev.x, p.py - ev.y)],
               1u);
   }
   if (REC) {
@@ -771,7 +784,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
   if (run[1]) flush_run<MULTI>(geoms, events, run[0], run[1]);
   __syncthreads();
   for (int i = threadIdx.x; i < a.n_geoms * kCounters; i += blockDim.x)
-    if (s_cnt[i]) atomicAdd(&a.counters[i], (unsigned long long)s_cnt[i]);
+    if (s_cnt[i]) tally_add(&a.counters[i], (unsigned long long)s_cnt[i]);
   for (int t = 0; t < a.n_tiles; ++t) {
     const GeomDev& gt = geoms[t];
     const int i0 = tile_origin(gt.xdim, side), j0 = tile_origin(gt.ydim, side);
@@ -779,7 +792,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
       const uint32_t v = s_tile[t * tile_area + e];
       if (v) {
         const int ti = e / side;
-        atomicAdd(&gt.prf[(size_t)(i0 + ti) * gt.ydim + (j0 + e - ti * side)],
+        tally_add(&gt.prf[(size_t)(i0 + ti) * gt.ydim + (j0 + e - ti * side)],
                   (unsigned long long)v);
       }
     }
