@@ -191,57 +191,77 @@ __global__ void k_pack_inputs(const float* x, int n, uint16_t* x0) {
   x0[(size_t)row * kInPad + c] = c < kFeat ? f2bf(x[(size_t)row * kFeat + c]) : (uint16_t)0;
 }
 
-// gather a minibatch: rows idx[b] of the resident data set
+// gather a minibatch: rows idx[b] of the resident data set, one wave per
+// row (lanes over the row's columns, so the copies are coalesced)
 __global__ void k_gather(const float* xs, const float* ys, const uint32_t* idx, int B,
                          uint16_t* x0, float* y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (i >= B) return;
   const uint32_t r = idx[i];
-  for (int c = 0; c < kInPad; ++c)
-    x0[(size_t)i * kInPad + c] = c < kFeat ? f2bf(xs[(size_t)r * kFeat + c]) : (uint16_t)0;
-  for (int c = 0; c < kTarget; ++c) y[(size_t)i * kTarget + c] = ys[(size_t)r * kTarget + c];
+  if (lane < kInPad)
+    x0[(size_t)i * kInPad + lane] = lane < kFeat ? f2bf(xs[(size_t)r * kFeat + lane]) : (uint16_t)0;
+  for (int c = lane; c < kTarget; c += 64) y[(size_t)i * kTarget + c] = ys[(size_t)r * kTarget + c];
 }
 
-// loss and dL/dZ4 for one sample per wave: lane k handles radial logit k
-// (z[1+k]); lane 0 also the detection logit.  dz -> bf16 [B x 80];
-// column 0 and 1..64 bias gradients accumulated.
+// loss and dL/dZ4, one sample at a time per wave: lane k handles radial
+// logit k (z[1+k]); lane 0 also the detection logit.  dz -> bf16 [B x 80].
+// The column 0 and 1..64 bias gradients and the loss are summed over the
+// wave's samples in registers, then over the block's waves in LDS, and
+// added with one atomic per column and block: one atomic per sample and
+// column made 4096 x 65 float atomics meet on 65 addresses (142 us per
+// batch of 4096); 32 blocks of long per-wave loops took 69 us.
+constexpr int kHeadLossBlocks = 256;
 __global__ void __launch_bounds__(256) k_head_loss(const float* z4, const float* y, int B,
                                                    uint16_t* dz, float* db3, float* loss) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= B) return;
-  const float* zr = z4 + (size_t)wave * kOutPad;
-  const float* yr = y + (size_t)wave * kTarget;
+  __shared__ float s_db[4][kOut];
+  __shared__ float s_loss[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = gridDim.x * (blockDim.x >> 6);
   const float invB = 1.0f / (float)B;
-  // radial softmax over 64 lanes
-  const float zl = zr[1 + lane];
-  float mx = zl;
-  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-  const float e = expf(zl - mx);
-  float se = e;
-  for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
-  const float prob = e / se;
-  const float q = yr[1 + lane];
-  float qs = q;
-  for (int o = 32; o > 0; o >>= 1) qs += __shfl_xor(qs, o, 64);
-  const float wrad = qs > 0.0f ? 1.0f : 0.0f;
-  float ce = q > 0.0f ? -q * (zl - mx - logf(se)) : 0.0f;
-  for (int o = 32; o > 0; o >>= 1) ce += __shfl_xor(ce, o, 64);
-  const float gr = wrad * (prob - q) * invB;
-  uint16_t* dr = dz + (size_t)wave * kOutPad;
-  dr[1 + lane] = f2bf(gr);
-  atomicAdd(&db3[1 + lane], bf2f(f2bf(gr)));
-  if (lane < kOutPad - 1 - 64) dr[65 + lane] = 0;
-  if (lane == 0) {
-    const float z0 = zr[0], y0 = yr[0];
-    const float p = 1.0f / (1.0f + expf(-z0));
-    // BCE with logits: max(z,0) - z y + log(1 + e^-|z|)
-    const float bce = fmaxf(z0, 0.0f) - z0 * y0 + log1pf(expf(-fabsf(z0)));
-    const float g0 = (p - y0) * invB;
-    dr[0] = f2bf(g0);
-    atomicAdd(&db3[0], bf2f(f2bf(g0)));
-    atomicAdd(loss, (bce + wrad * ce) * invB);
+  float db_r = 0.0f, db_0 = 0.0f, lsum = 0.0f;
+  for (int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; wave < B; wave += nw) {
+    const float* zr = z4 + (size_t)wave * kOutPad;
+    const float* yr = y + (size_t)wave * kTarget;
+    // radial softmax over 64 lanes
+    const float zl = zr[1 + lane];
+    float mx = zl;
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    const float e = expf(zl - mx);
+    float se = e;
+    for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
+    const float prob = e / se;
+    const float q = yr[1 + lane];
+    float qs = q;
+    for (int o = 32; o > 0; o >>= 1) qs += __shfl_xor(qs, o, 64);
+    const float wrad = qs > 0.0f ? 1.0f : 0.0f;
+    float ce = q > 0.0f ? -q * (zl - mx - logf(se)) : 0.0f;
+    for (int o = 32; o > 0; o >>= 1) ce += __shfl_xor(ce, o, 64);
+    const float gr = wrad * (prob - q) * invB;
+    uint16_t* dr = dz + (size_t)wave * kOutPad;
+    const uint16_t grb = f2bf(gr);
+    dr[1 + lane] = grb;
+    db_r += bf2f(grb);
+    if (lane < kOutPad - 1 - 64) dr[65 + lane] = 0;
+    if (lane == 0) {
+      const float z0 = zr[0], y0 = yr[0];
+      const float p = 1.0f / (1.0f + expf(-z0));
+      // BCE with logits: max(z,0) - z y + log(1 + e^-|z|)
+      const float bce = fmaxf(z0, 0.0f) - z0 * y0 + log1pf(expf(-fabsf(z0)));
+      const uint16_t g0 = f2bf((p - y0) * invB);
+      dr[0] = g0;
+      db_0 += bf2f(g0);
+      lsum += (bce + wrad * ce) * invB;
+    }
   }
+  s_db[w][1 + lane] = db_r;
+  if (lane == 0) {
+    s_db[w][0] = db_0;
+    s_loss[w] = lsum;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < kOut) atomicAdd(&db3[t], (s_db[0][t] + s_db[1][t]) + (s_db[2][t] + s_db[3][t]));
+  if (t == 0) atomicAdd(loss, (s_loss[0] + s_loss[1]) + (s_loss[2] + s_loss[3]));
 }
 
 // logits -> [p_det, radial probabilities] (FP32 [n x 65])
@@ -289,6 +309,140 @@ __global__ void k_zero(float* p, size_t n) {
     p[i] = 0.0f;
 }
 
+// ------------------------------------------------- fused inference forward
+
+// Weight matrices in MFMA fragment order for the fused forward: layer l's
+// W [K x N] (row-major at kWOff[l] in the bf16 copy) is stored at the same
+// offset as the B fragments of v_mfma_f32_16x16x16bf16_1k, column block cb
+// (16 columns) by k step ks (16 k) by lane by element: lane 16 q + c holds
+// W[16 ks + 4 q + e][16 cb + c], e = 0..3.  One fragment is then 512
+// contiguous bytes, a lane's part one 8-byte load.  Bias slots are unused.
+__global__ void k_fragment_weights(const uint16_t* __restrict__ wb, uint16_t* __restrict__ wt) {
+  const int ks[4] = {kInPad, kHidden, kHidden, kHidden};
+  const int ns[4] = {kHidden, kHidden, kHidden, kOutPad};
+  const int l = blockIdx.y;
+  const int K = ks[l], N = ns[l];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < K * N; i += gridDim.x * blockDim.x) {
+    const int k = i / N, n = i - k * N;
+    const int lane = 16 * ((k & 15) >> 2) + (n & 15);
+    const size_t frag = (size_t)(n >> 4) * (K >> 4) + (k >> 4);
+    wt[kWOff[l] + (frag * 64 + lane) * 4 + (k & 3)] = wb[kWOff[l] + i];
+  }
+}
+
+// The inference forward with all four layers in one kernel: a workgroup
+// of 4 waves takes kFuseRows = 64 rows, keeps their activations in LDS
+// (bf16, 64 x 256) from layer to layer and writes only the output logits
+// (FP32, B x kOutPad), where the layered forward writes every layer's FP32
+// pre-activations and bf16 activations to HBM and reads them back.
+// Hidden layers: wave w computes the 64 rows x columns [64w, 64w + 64) as
+// 4 x 4 tiles of v_mfma_f32_16x16x16bf16_1k, A fragments from LDS and B
+// fragments straight from the fragment-ordered weights (312 KB in all, read by
+// every workgroup, so they stay in L2); after a barrier the epilogue
+// (bias, GELU, bf16) overwrites the activations in place.  Output layer:
+// wave w takes rows [16w, 16w + 16) and the 5 column tiles.  Fragments,
+// the k order of the accumulation and the epilogue arithmetic are those of
+// k_gemm, so the logits equal the layered forward's bit for bit.
+constexpr int kFuseRows = 64;
+constexpr int kActLd = kHidden + 8;  // LDS row stride (bf16): 528 B
+
+__global__ void __launch_bounds__(256) k_forward_fused(const uint16_t* __restrict__ x0, int n,
+                                                       const uint16_t* __restrict__ wt,
+                                                       const float* __restrict__ w,
+                                                       float* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) uint16_t act[kFuseRows][kActLd];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int row0 = blockIdx.x * kFuseRows;
+  // input rows (32 bf16 each) -> act[r][0 .. 31]; rows past n are zero
+  for (int i = t; i < kFuseRows * (kInPad / 4); i += 256) {
+    const int r = i / (kInPad / 4), c = (i % (kInPad / 4)) * 4;
+    uint2 v = make_uint2(0u, 0u);
+    if (row0 + r < n) v = *reinterpret_cast<const uint2*>(x0 + (size_t)(row0 + r) * kInPad + c);
+    *reinterpret_cast<uint2*>(&act[r][c]) = v;
+  }
+  __syncthreads();
+  const int lr = lane & 15, lq = lane >> 4;
+  for (int l = 0; l < 3; ++l) {
+    const int K = l == 0 ? kInPad : kHidden;
+    const int n0 = wv * 64;
+    const int steps = K / 16;  // 2 or 16
+    // fragment (column block 4 wv + j, k step s) at wl + (j steps + s) 256
+    const uint16_t* wl = wt + kWOff[l] + (size_t)(4 * wv * steps) * 256 + 4 * lane;
+    float4_ acc[4][4];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) acc[i][j] = float4_{0.f, 0.f, 0.f, 0.f};
+    // Weight fragments come from L2 and load kPre k steps ahead: a ring of
+    // kPre x 4 fragments, the k loop unrolled by kPre so that the ring
+    // index is a constant.  With the fragment-ordered weights (one 512-byte
+    // read per fragment) depths 2, 4 and 8 time the same on MI355X (0.078
+    // to 0.080 ms per 65,536 rows); with one column per lane from the
+    // row-major transpose (16 partial cache lines per fragment) the forward
+    // took 0.17, 0.16 and 0.14 ms.
+    constexpr int kPre = 2;
+    short4_ bq[kPre][4];
+    for (int d = 0; d < kPre; ++d)
+      for (int j = 0; j < 4; ++j)
+        bq[d][j] = d < steps ? *reinterpret_cast<const short4_*>(wl + (size_t)(j * steps + d) * 256)
+                             : short4_{0, 0, 0, 0};
+    for (int s0 = 0; s0 < steps; s0 += kPre) {
+#pragma unroll
+      for (int d = 0; d < kPre; ++d) {
+        const int s = s0 + d;
+        if (s < steps) {
+          short4_ af[4];
+          for (int i = 0; i < 4; ++i)
+            af[i] = *reinterpret_cast<const short4_*>(&act[i * 16 + lr][16 * s + 4 * lq]);
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+              acc[i][j] =
+                  __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bq[d][j], acc[i][j], 0, 0, 0);
+          if (s + kPre < steps)
+            for (int j = 0; j < 4; ++j)
+              bq[d][j] = *reinterpret_cast<const short4_*>(wl + (size_t)(j * steps + s + kPre) * 256);
+        }
+      }
+    }
+    __syncthreads();  // every wave has read this layer's inputs
+    const float* b = w + kBOff[l];
+    for (int j = 0; j < 4; ++j) {
+      const int col = n0 + j * 16 + lr;
+      const float bj = b[col];
+      for (int i = 0; i < 4; ++i)
+        for (int r = 0; r < 4; ++r) act[i * 16 + 4 * lq + r][col] = f2bf(gelu_f(acc[i][j][r] + bj));
+    }
+    __syncthreads();
+  }
+  const uint16_t* wl = wt + kWOff[3] + 4 * lane;  // fragment (j, s) at wl + (16 j + s) 256
+  float4_ acc[5];
+  for (int j = 0; j < 5; ++j) acc[j] = float4_{0.f, 0.f, 0.f, 0.f};
+  constexpr int kPre = 4, kSteps = kHidden / 16;
+  short4_ bq[kPre][5];
+  for (int d = 0; d < kPre; ++d)
+    for (int j = 0; j < 5; ++j)
+      bq[d][j] = *reinterpret_cast<const short4_*>(wl + (size_t)(j * kSteps + d) * 256);
+  for (int s0 = 0; s0 < kSteps; s0 += kPre) {
+#pragma unroll
+    for (int d = 0; d < kPre; ++d) {
+      const int s = s0 + d;
+      const short4_ af = *reinterpret_cast<const short4_*>(&act[wv * 16 + lr][16 * s + 4 * lq]);
+      for (int j = 0; j < 5; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bq[d][j], acc[j], 0, 0, 0);
+      if (s + kPre < kSteps)
+        for (int j = 0; j < 5; ++j)
+          bq[d][j] = *reinterpret_cast<const short4_*>(wl + (size_t)(j * kSteps + s + kPre) * 256);
+    }
+  }
+  const float* b = w + kBOff[3];
+  for (int j = 0; j < 5; ++j) {
+    const int col = j * 16 + lr;
+    const float bj = b[col];
+    for (int r = 0; r < 4; ++r) {
+      const int row = row0 + wv * 16 + 4 * lq + r;
+      if (row < n) out[(size_t)row * kOutPad + col] = acc[j][r] + bj;
+    }
+  }
+}
+
 }  // namespace fd2mlp
 
 using namespace fd2mlp;
@@ -304,6 +458,8 @@ struct fd2_mlp {
   float* m = nullptr;   // Adam moments
   float* v = nullptr;
   uint16_t* wb = nullptr;  // bf16 copy (same layout as w)
+  uint16_t* wt = nullptr;  // bf16 weight matrices transposed (fused forward)
+  bool wt_stale = true;    // wb changed since wt was built
   uint16_t* x0 = nullptr;
   float* z[4] = {};
   uint16_t* a[3] = {};
@@ -385,6 +541,23 @@ int forward(fd2_mlp* p, int B, hipStream_t stream, float* out = nullptr) {
   return FD2_OK;
 }
 
+// inference forward of B rows of p->x0 in one kernel (k_forward_fused):
+// the output logits go to `out` (B x kOutPad f32) when given, else to
+// p->z[3]; the other layers' buffers are not written.  Rebuilds the
+// transposed weights first when the parameters changed.
+int forward_fused(fd2_mlp* p, int B, hipStream_t stream, float* out = nullptr) {
+  if (p->wt_stale) {
+    hipLaunchKernelGGL(k_fragment_weights, dim3(64, 4), dim3(256), 0, stream, p->wb, p->wt);
+    MLP_HIP(hipGetLastError());
+    p->wt_stale = false;
+  }
+  if (B <= 0) return FD2_OK;
+  hipLaunchKernelGGL(k_forward_fused, dim3((unsigned)((B + kFuseRows - 1) / kFuseRows)),
+                     dim3(256), 0, stream, p->x0, B, p->wt, p->w, out ? out : p->z[3]);
+  MLP_HIP(hipGetLastError());
+  return FD2_OK;
+}
+
 // backward from the bf16 output gradient in p->dz (B x 80) into p->g
 int backward(fd2_mlp* p, int B) {
   const int ins[4] = {kInPad, kHidden, kHidden, kHidden};
@@ -434,7 +607,8 @@ int backward(fd2_mlp* p, int B) {
 
 int loss_grad(fd2_mlp* p, int B, float* d_loss) {
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, p->stream, d_loss, (size_t)1);
-  hipLaunchKernelGGL(k_head_loss, dim3((B * 64 + 255) / 256), dim3(256), 0, p->stream,
+  hipLaunchKernelGGL(k_head_loss, dim3(std::min((B * 64 + 255) / 256, kHeadLossBlocks)), dim3(256),
+                     0, p->stream,
                      p->z[3], p->y, B, p->dz, p->g + kBOff[3], d_loss);
   MLP_HIP(hipGetLastError());
   return FD2_OK;
@@ -487,7 +661,8 @@ int fd2_mlp_create(int device, uint32_t seed, fd2_mlp** out) {
     const size_t n = kParams;
     if (hipMalloc(&p->w, n * 4) != hipSuccess || hipMalloc(&p->g, n * 4) != hipSuccess ||
         hipMalloc(&p->m, n * 4) != hipSuccess || hipMalloc(&p->v, n * 4) != hipSuccess ||
-        hipMalloc(&p->wb, n * 2) != hipSuccess || hipMalloc(&p->loss, 4) != hipSuccess) {
+        hipMalloc(&p->wb, n * 2) != hipSuccess || hipMalloc(&p->wt, n * 2) != hipSuccess ||
+        hipMalloc(&p->loss, 4) != hipSuccess) {
       rc = FD2_E_NOMEM;
       break;
     }
@@ -508,7 +683,7 @@ void fd2_mlp_destroy(fd2_mlp* p) {
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   free_batch(p);
-  void* bufs[] = {p->w, p->g, p->m, p->v, p->wb, p->loss, p->data_x, p->data_y};
+  void* bufs[] = {p->w, p->g, p->m, p->v, p->wb, p->wt, p->loss, p->data_x, p->data_y};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (p->e0) (void)hipEventDestroy(p->e0);
@@ -523,6 +698,7 @@ int fd2_mlp_set_params(fd2_mlp* p, const float* host, size_t n) {
   MLP_HIP(hipMemcpyAsync(p->w, host, n * 4, hipMemcpyHostToDevice, p->stream));
   hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
                      p->w, p->wb, (int)n);
+  p->wt_stale = true;
   const unsigned nb = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, p->stream, p->g, n);
   hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, p->stream, p->m, n);
@@ -563,7 +739,7 @@ int fd2_mlp_forward(fd2_mlp* p, const float* x, int64_t n, float* out) {
     }
     hipLaunchKernelGGL(k_pack_inputs, dim3((b * kIn + 255) / 256), dim3(256), 0, p->stream,
                        dx, b, p->x0);
-    rc = forward(p, b, p->stream);
+    rc = forward_fused(p, b, p->stream);
     if (rc) break;
     hipLaunchKernelGGL(k_head_probs, dim3((b * 64 + 255) / 256), dim3(256), 0, p->stream,
                        p->z[3], b, dout);
@@ -577,16 +753,45 @@ int fd2_mlp_forward(fd2_mlp* p, const float* x, int64_t n, float* out) {
   return rc;
 }
 
+int fd2_mlp_logits(fd2_mlp* p, const float* x, int64_t n, float* logits, int layered) {
+  if (!p || n < 0 || (n > 0 && (!x || !logits))) return FD2_E_INVAL;
+  MLP_HIP(hipSetDevice(p->device));
+  const int64_t chunk = 65536;
+  float* dx = nullptr;
+  int rc = ensure_batch(p, (int)chunk);
+  if (rc) return rc;
+  MLP_HIP(hipMalloc(&dx, (size_t)chunk * kFeat * 4));
+  for (int64_t s = 0; s < n && rc == FD2_OK; s += chunk) {
+    const int b = (int)(n - s < chunk ? n - s : chunk);
+    if (hipMemcpyAsync(dx, x + s * kFeat, (size_t)b * kFeat * 4, hipMemcpyHostToDevice,
+                       p->stream) != hipSuccess) {
+      rc = FD2_E_DEVICE;
+      break;
+    }
+    hipLaunchKernelGGL(k_pack_inputs, dim3((b * kIn + 255) / 256), dim3(256), 0, p->stream,
+                       dx, b, p->x0);
+    rc = layered ? forward(p, b, p->stream) : forward_fused(p, b, p->stream);
+    if (rc) break;
+    // the kOut logits of each row (of kOutPad computed)
+    if (hipMemcpy2DAsync(logits + s * kOut, kOut * 4, p->z[3], kOutPad * 4, kOut * 4, b,
+                         hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
+      rc = FD2_E_DEVICE;
+  }
+  (void)hipFree(dx);
+  return rc;
+}
+
 int fd2_mlp_time_forward(fd2_mlp* p, int B, int reps, float* ms) {
   if (!p || !ms || B <= 0 || B % 16 || reps <= 0) return FD2_E_INVAL;
   MLP_HIP(hipSetDevice(p->device));
   int rc = ensure_batch(p, B);
   if (rc) return rc;
   MLP_HIP(hipMemsetAsync(p->x0, 0, (size_t)B * kInPad * 2, p->stream));  // defined inputs
-  rc = forward(p, B, p->stream);  // warm-up
+  rc = forward_fused(p, B, p->stream);  // warm-up
   if (rc) return rc;
   MLP_HIP(hipEventRecord(p->e0, p->stream));
-  for (int r = 0; r < reps && rc == FD2_OK; ++r) rc = forward(p, B, p->stream);
+  for (int r = 0; r < reps && rc == FD2_OK; ++r) rc = forward_fused(p, B, p->stream);
   if (rc) return rc;
   MLP_HIP(hipEventRecord(p->e1, p->stream));
   MLP_HIP(hipStreamSynchronize(p->stream));
@@ -669,7 +874,7 @@ int fd2_mlp_train(fd2_mlp* p, int steps, int B, float lr, uint32_t seed, float* 
            ? FD2_OK
            : FD2_E_DEVICE;
   for (int st = 0; st < steps && rc == FD2_OK; ++st) {
-    hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, p->stream, p->data_x,
+    hipLaunchKernelGGL(k_gather, dim3((B * 64 + 255) / 256), dim3(256), 0, p->stream, p->data_x,
                        p->data_y, d_idx + (size_t)st * B, B, p->x0, p->y);
     rc = forward(p, B, p->stream);
     if (!rc) rc = loss_grad(p, B, d_losses + st);
@@ -679,6 +884,7 @@ int fd2_mlp_train(fd2_mlp* p, int steps, int B, float lr, uint32_t seed, float* 
     const float c1 = 1.0f - std::pow(b1, (float)p->step), c2 = 1.0f - std::pow(b2, (float)p->step);
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((kParams + 255) / 256)), dim3(256), 0, p->stream,
                        p->w, p->g, p->m, p->v, p->wb, (int)kParams, lr, b1, b2, eps, c1, c2);
+    p->wt_stale = true;
     if (hipGetLastError() != hipSuccess) rc = FD2_E_DEVICE;
   }
   if (!rc && (hipEventRecord(p->e1, p->stream) != hipSuccess ||
@@ -1154,7 +1360,7 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
       const int b = (int)(n_audit - s0 < chunk ? n_audit - s0 : chunk);
       hipLaunchKernelGGL(k_pack_inputs, dim3((b * kIn + 255) / 256), dim3(256), 0, p->stream,
                          d_feat + (size_t)s0 * kFeat, b, p->x0);
-      rc = forward(p, b, p->stream);
+      rc = forward_fused(p, b, p->stream);
       if (rc) break;
       hipLaunchKernelGGL(k_head_probs, dim3((b * 64 + 255) / 256), dim3(256), 0, p->stream,
                          p->z[3], b, d_pred + (size_t)s0 * kTarget);
@@ -1228,7 +1434,7 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
       const int b = (int)(ns - s0 < chunk ? ns - s0 : chunk);
       hipLaunchKernelGGL(fd2mlp::k_surr_features, dim3((b * kInPad + 255) / 256), dim3(256), 0,
                          p->stream, d_ev + s0, b, c->d_geoms, p->x0);
-      rc = forward(p, b, p->stream, d_logits + (size_t)s0 * kOutPad);
+      rc = forward_fused(p, b, p->stream, d_logits + (size_t)s0 * kOutPad);
     }
     for (size_t k = 0; k < slice_start.size() && rc == FD2_OK; ++k) {
       const int64_t s0 = slice_start[k];

@@ -403,6 +403,7 @@ def _mlp_lib() -> C.CDLL:
             "fd2_mlp_set_params": ([P, P, C.c_size_t], C.c_int),
             "fd2_mlp_get_params": ([P, P, C.c_size_t], C.c_int),
             "fd2_mlp_forward": ([P, P, C.c_int64, P], C.c_int),
+            "fd2_mlp_logits": ([P, P, C.c_int64, P, C.c_int], C.c_int),
             "fd2_mlp_gradients": ([P, P, P, C.c_int, P, C.POINTER(C.c_float)], C.c_int),
             "fd2_mlp_set_data": ([P, P, P, C.c_int64], C.c_int),
             "fd2_mlp_train": ([P, C.c_int, C.c_int, C.c_float, C.c_uint32, P,
@@ -464,6 +465,15 @@ class Mlp:
                "mlp_forward")
         return out
 
+    def logits(self, x, layered: bool = False) -> np.ndarray:
+        """Raw output logits [n, 65] (fd2_mlp_logits): the fused inference
+        forward, or with layered=True the per-layer GEMMs training uses."""
+        a = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((len(a), self.TARGETS), np.float32)
+        _check(_mlp_lib().fd2_mlp_logits(self._h, a.ctypes.data, len(a), out.ctypes.data,
+                                         1 if layered else 0), "mlp_logits")
+        return out
+
     def gradients(self, x, y):
         a = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(y, np.float32)
@@ -481,8 +491,8 @@ class Mlp:
                "mlp_set_data")
 
     def time_forward(self, batch: int = 65536, reps: int = 20) -> float:
-        """Mean GPU ms of the forward GEMMs on a device-resident batch
-        (fd2_mlp_time_forward)."""
+        """Mean GPU ms of the fused inference forward on a device-resident
+        batch (fd2_mlp_time_forward)."""
         ms = C.c_float()
         _check(_mlp_lib().fd2_mlp_time_forward(self._h, batch, reps, C.byref(ms)),
                "mlp_time_forward")

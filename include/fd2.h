@@ -295,6 +295,12 @@ int fd2_mlp_set_params(fd2_mlp *mlp, const float *params, size_t n);
 int fd2_mlp_get_params(fd2_mlp *mlp, float *params, size_t n);
 /* probabilities for n host samples: out[n][FD2_MLP_TARGETS] */
 int fd2_mlp_forward(fd2_mlp *mlp, const float *x, int64_t n, float *out);
+/* the network's raw output logits for n host samples, logits[n][65]
+ * (detection logit, then the 64 radial logits): layered = 0 runs the fused
+ * inference forward (one kernel for the four layers, the path inference
+ * and the surrogate transport use), layered = 1 the per-layer GEMMs that
+ * training uses; the two give the same bits */
+int fd2_mlp_logits(fd2_mlp *mlp, const float *x, int64_t n, float *logits, int layered);
 /* gradient of the mean loss over `batch` samples (batch % 16 == 0), no
  * update: grads[fd2_mlp_num_params()], *loss */
 int fd2_mlp_gradients(fd2_mlp *mlp, const float *x, const float *y, int batch,
@@ -304,11 +310,11 @@ int fd2_mlp_gradients(fd2_mlp *mlp, const float *x, const float *y, int batch,
 int fd2_mlp_set_data(fd2_mlp *mlp, const float *x, const float *y, int64_t n);
 int fd2_mlp_train(fd2_mlp *mlp, int steps, int batch, float lr, uint32_t seed,
                   float *losses, float *ms);
-/* Measurement: the network's forward GEMMs (4 layers, BF16 MFMA, FP32
- * accumulate, bias + GELU epilogues) on a device-resident batch of `batch`
- * rows (multiple of 16, the inputs currently in the network's batch
- * buffer), `reps` times; *ms is the mean GPU time of one forward (no host
- * copies).  2 x 148,992 FLOP per row. */
+/* Measurement: the fused inference forward (4 layers in one kernel, BF16
+ * MFMA, FP32 accumulate, bias + GELU epilogues) on a device-resident batch
+ * of `batch` rows (multiple of 16, the inputs currently in the network's
+ * batch buffer), `reps` times; *ms is the mean GPU time of one forward (no
+ * host copies).  2 x 148,992 FLOP per row. */
 int fd2_mlp_time_forward(fd2_mlp *mlp, int batch, int reps, float *ms);
 
 /* Surrogate transport with plain-MC fallback.  Each event goes to the plain

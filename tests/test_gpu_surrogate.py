@@ -30,6 +30,36 @@ def test_forward_matches_reference(fd2lib):
     assert np.allclose(out[:, 1:].sum(1), 1.0, atol=1e-5)
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 300, 4097, 70000])
+def test_fused_forward_equals_layered_forward(fd2lib, n):
+    """The fused inference kernel (four layers, activations in LDS) gives
+    the per-layer GEMM path's logits bit for bit: same MFMA fragments, same
+    k order, same epilogue arithmetic.  Row counts around the 64-row
+    workgroup tile and past the 65,536-row host batch; trained-like
+    parameters (non-zero biases everywhere); the fused path rebuilds its
+    transposed weights after set_params and after training steps."""
+    rng = np.random.default_rng(n)
+    p = M.init_params(11)
+    for l in range(4):
+        p[M.B_OFF[l]:M.B_OFF[l] + M.SHAPES[l][1]] = rng.normal(0, 0.2, M.SHAPES[l][1])
+    x = rng.random((n, 5)).astype(np.float32) * 2.0 - 0.5
+    with fd2lib.Mlp() as net:
+        net.set_params(p)
+        fused, layered = net.logits(x), net.logits(x, layered=True)
+        assert np.array_equal(fused, layered)
+        assert np.isfinite(fused).all() and np.abs(fused).max() > 0
+        # the reference at FP32-accumulation tolerance, through the probabilities
+        if n <= 4097:
+            assert np.abs(net.forward(x) - M.probs(p, x)).max() < 2e-3
+        if n == 300:  # after Adam steps the transposed copy is rebuilt
+            xs, ys = batch(rng, 1024)
+            net.set_data(xs, ys)
+            net.train(3, batch=256, lr=1e-2)
+            f2, l2 = net.logits(x), net.logits(x, layered=True)
+            assert np.array_equal(f2, l2)
+            assert not np.array_equal(f2, fused)
+
+
 def test_gradients_match_reference(fd2lib):
     rng = np.random.default_rng(1)
     p = M.init_params(7)
