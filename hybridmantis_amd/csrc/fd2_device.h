@@ -166,6 +166,19 @@ __device__ __forceinline__ void ranecu_init(RanecuRng& g, int batch,
 
 // ----------------------------------------------------------- tallies
 
+// Tally atomics (PRF, history counts, radial bins) through
+// global (address space 1) pointers.  The tally pointers are read from the
+// geometry table, so the compiler cannot tell their address space and would
+// issue flat atomics; a flat operation counts against lgkmcnt as well as
+// vmcnt, and the wave's next wait for an LDS or scalar load would then wait
+// for the atomic to reach L2 (on every detection, at the pencil beam's few
+// hot pixels).  A global atomic without return counts against vmcnt only.
+template <class T>
+__device__ __forceinline__ void tally_add(T* p, T v) {
+  __hip_atomic_fetch_add((__attribute__((address_space(1))) T*)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
 enum Fate : int {
   kNone = 0, kDetect = 1, kDetectOut = 2, kAbsTop = 3, kAbsBulk = 4,
   kLost = 5, kOutOfCol = 6, kTheta1 = 7,

@@ -473,6 +473,13 @@ struct fd2_mlp {
   uint32_t* idx = nullptr;
   int64_t step = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // fd2_surrogate_submit's buffers, kept between calls and grown on demand
+  // (a call of the same size allocates nothing): device buffers by SurrBuf
+  // index, and the pinned copy of the audit predictions
+  void* sbuf[9] = {};
+  size_t sbuf_cap[9] = {};
+  float* h_pred = nullptr;
+  size_t h_pred_cap = 0;
 };
 
 namespace {
@@ -494,6 +501,29 @@ void free_batch(fd2_mlp* p) {
   p->y = nullptr;
   p->idx = nullptr;
   p->cap = 0;
+}
+
+// fd2_surrogate_submit's device buffers (fd2_mlp::sbuf)
+enum SurrBuf : int {
+  kBufAll, kBufEv, kBufIdx, kBufIds, kBufRad, kBufLogits, kBufSpread, kBufFeat, kBufPred, kBufCount
+};
+static_assert(kBufCount == sizeof(fd2_mlp::sbuf) / sizeof(void*), "one slot per buffer");
+
+// Buffer `which` with room for `bytes`: the kept one, or a larger one (the
+// old one freed; the previous call has synchronised its streams).
+template <class T>
+bool reserve(fd2_mlp* p, int which, size_t bytes, T** out) {
+  void*& b = p->sbuf[which];
+  size_t& cap = p->sbuf_cap[which];
+  if (!b || cap < bytes) {
+    if (b) (void)hipFree(b);
+    b = nullptr;
+    cap = 0;
+    if (hipMalloc(&b, bytes ? bytes : 1) != hipSuccess) return false;
+    cap = bytes;
+  }
+  *out = static_cast<T*>(b);
+  return true;
 }
 
 int ensure_batch(fd2_mlp* p, int B) {
@@ -686,6 +716,9 @@ void fd2_mlp_destroy(fd2_mlp* p) {
   void* bufs[] = {p->w, p->g, p->m, p->v, p->wb, p->wt, p->loss, p->data_x, p->data_y};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (void* b : p->sbuf)
+    if (b) (void)hipFree(b);
+  if (p->h_pred) (void)hipHostFree(p->h_pred);
   if (p->e0) (void)hipEventDestroy(p->e0);
   if (p->e1) (void)hipEventDestroy(p->e1);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -1009,9 +1042,21 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
               const fd2::GeomDev* __restrict__ geoms) {
   extern __shared__ uint32_t s_tile[];
   __shared__ uint32_t s_cnt[FD2_MAX_GEOMS * 3];
+  // the radial bins' edges and area centroids (constants), and per wave its
+  // event's CDF, slopes and guide table (s_guide[k]: the first bin whose CDF
+  // reaches k/64), so that a detection finds its bin, slope and edges with
+  // a few independent LDS reads instead of six dependent shuffles and four
+  // exponentials
+  __shared__ float s_edge[FD2_RADIAL_BINS + 1];
+  __shared__ float s_cent[FD2_RADIAL_BINS];
+  __shared__ float s_cdf[THREADS / 64][64];
+  __shared__ float s_slope[THREADS / 64][64];
+  __shared__ int s_guide[THREADS / 64][64];
   const int T = a.tile, T2 = a.tile * a.tile;
   for (int i = threadIdx.x; i < T2; i += blockDim.x) s_tile[i] = 0u;
   for (int i = threadIdx.x; i < a.n_geoms * 3; i += blockDim.x) s_cnt[i] = 0u;
+  if (threadIdx.x <= FD2_RADIAL_BINS) s_edge[threadIdx.x] = radial_edge(threadIdx.x);
+  if (threadIdx.x < FD2_RADIAL_BINS) s_cent[threadIdx.x] = radial_centroid(threadIdx.x);
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int waves = gridDim.x * (THREADS / 64);
@@ -1033,12 +1078,12 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
     // this lane's bin: mean density and relative slope within the bin
     float slope = 0.0f;
     {
-      const float e0 = radial_edge(lane), e1 = radial_edge(lane + 1);
+      const float e0 = s_edge[lane], e1 = s_edge[lane + 1];
       const float rho = (ex / total) / (e1 * e1 - e0 * e0);  // x pi, cancels
       const float rho_dn = __shfl_up(rho, 1, 64), rho_up = __shfl_down(rho, 1, 64);
-      const float c = radial_centroid(lane);
-      const float c_dn = lane > 0 ? radial_centroid(lane - 1) : 0.0f;
-      const float c_up = lane < 63 ? radial_centroid(lane + 1) : 0.0f;
+      const float c = s_cent[lane];
+      const float c_dn = lane > 0 ? s_cent[lane - 1] : 0.0f;
+      const float c_up = lane < 63 ? s_cent[lane + 1] : 0.0f;
       if (lane > 0 && lane < 63 && rho > 0.0f) {
         slope = (rho_up - rho_dn) / (rho * (c_up - c_dn));
         // keep 1 + slope (r - c) >= 0 on [e0, e1]
@@ -1046,6 +1091,22 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
         slope = fminf(fmaxf(slope, -lim_hi), lim_lo);
       }
     }
+    // the CDF is non-decreasing (an inclusive scan of non-negative terms
+    // over one divisor, the last set to 1), so the first bin reaching t is
+    // the number of bins below t
+    s_cdf[w][lane] = cdf;
+    s_slope[w][lane] = slope;
+    {
+      int gk = 0;
+      for (int k = 0; k < 64; ++k) {
+        const unsigned long long below = __ballot(cdf < (float)k * (1.0f / 64.0f));
+        if (lane == k) gk = (int)__popcll(below);
+      }
+      s_guide[w][lane] = gk;
+    }
+    // the wave's LDS writes complete in order before its reads below; the
+    // barrier keeps the compiler from moving the reads above them
+    __builtin_amdgcn_wave_barrier();
     const float pdet = 1.0f / (1.0f + expf(-z[0]));
     fd2::CounterRng rng;
     rng.seed(a.seed ^ 0x53555252u, ids[e], (uint32_t)lane);
@@ -1054,19 +1115,16 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
     const int K = cnt;
     for (int j0 = 0; j0 < K; j0 += 64) {
-      const bool act = j0 + lane < K;
-      const float u = act ? rng.u() : 2.0f;
-      // first bin whose CDF reaches u (binary search over the lanes' CDF)
-      int lo = 0;
-      for (int step = 32; step > 0; step >>= 1) {
-        const float c = __shfl(cdf, lo + step - 1, 64);
-        if (c < u) lo += step;
-      }
-      const int bin = lo > 63 ? 63 : lo;
-      const float sb = __shfl(slope, bin, 64);  // all lanes take part in the shuffle
-      if (!act) continue;
-      const float r0 = radial_edge(bin), r1 = radial_edge(bin + 1);
-      const float cb = radial_centroid(bin);
+      if (j0 + lane >= K) continue;
+      const float u = rng.u();  // in (0, 1]
+      // first bin whose CDF reaches u: from the guide entry of floor(64 u)
+      // (u >= k/64, so the bin is at or after it) forward; the last bin's
+      // CDF is 1, so the walk ends by bin 63
+      int bin = s_guide[w][min((int)(u * 64.0f), 63)];
+      while (s_cdf[w][bin] < u) ++bin;
+      const float sb = s_slope[w][bin];
+      const float r0 = s_edge[bin], r1 = s_edge[bin + 1];
+      const float cb = s_cent[bin];
       const float wmax = fmaxf(1.0f + sb * (r0 - cb), 1.0f + sb * (r1 - cb));
       float r;
       for (int tries = 0;; ++tries) {  // acceptance >= 1/2 per try
@@ -1085,14 +1143,14 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
           atomicAdd(&s_tile[ti * T + tj], 1u);
         else
           if (ev.geom == 0 && a.spread)
-            atomicAdd(&a.spread[(size_t)(blockIdx.x % kSpreadCopies) * g.xdim * g.ydim +
-                                (size_t)pi * g.ydim + pj], 1u);
+            fd2::tally_add(&a.spread[(size_t)(blockIdx.x % kSpreadCopies) * g.xdim * g.ydim +
+                                     (size_t)pi * g.ydim + pj], 1u);
           else
-            atomicAdd(&g.prf[(size_t)pi * g.ydim + pj], 1ull);
+            fd2::tally_add(&g.prf[(size_t)pi * g.ydim + pj], 1ull);
       }
     }
     if (lane == 0) {
-      if (ev.hist >= 1 && ev.hist <= g.num_hist && K > 0) atomicAdd(&g.hist[ev.hist - 1], (uint32_t)K);
+      if (ev.hist >= 1 && ev.hist <= g.num_hist && K > 0) fd2::tally_add(&g.hist[ev.hist - 1], (uint32_t)K);
       atomicAdd(&s_cnt[ev.geom * 3 + 0], (uint32_t)ev.N);
       atomicAdd(&s_cnt[ev.geom * 3 + 1], (uint32_t)K);
       atomicAdd(&s_cnt[ev.geom * 3 + 2], (uint32_t)(ev.N - K));
@@ -1105,7 +1163,7 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
       const uint32_t v = s_tile[i];
       if (!v) continue;
       const int pi = a.tile_i0 + i / T, pj = a.tile_j0 + i % T;
-      atomicAdd(&g0.prf[(size_t)pi * g0.ydim + pj], (unsigned long long)v);
+      fd2::tally_add(&g0.prf[(size_t)pi * g0.ydim + pj], (unsigned long long)v);
     }
   }
   for (int gi = threadIdx.x; gi < a.n_geoms; gi += blockDim.x) {
@@ -1307,14 +1365,11 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
   float* d_pred = nullptr;
   float* h_pred = nullptr;  // pinned
   std::vector<float> feat;
-  // waits for both streams before the buffers go
+  // waits for both streams before returning; the buffers stay with the
+  // network for its next call
   auto release = [&](int status) {
     if (hipStreamSynchronize(c->stream) != hipSuccess && !status) status = FD2_E_DEVICE;
     if (hipStreamSynchronize(p->stream) != hipSuccess && !status) status = FD2_E_DEVICE;
-    for (void* b : {(void*)d_all, (void*)d_ev, (void*)d_idx, (void*)d_ids, (void*)d_rad,
-                    (void*)d_logits, (void*)d_spread, (void*)d_feat, (void*)d_pred})
-      if (b) (void)hipFree(b);
-    if (h_pred) (void)hipHostFree(h_pred);
     return status;
   };
 
@@ -1325,7 +1380,7 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
   int rc = FD2_OK;
   mc.insert(mc.end(), audit.begin(), audit.end());
   if (!mc.empty()) {
-    if (hipMalloc(&d_rad, mc.size() * FD2_RADIAL_BINS * sizeof(uint32_t)) != hipSuccess)
+    if (!reserve(p, kBufRad, mc.size() * FD2_RADIAL_BINS * sizeof(uint32_t), &d_rad))
       return release(FD2_E_NOMEM);
     rc = fd2::submit_radial_async(c, mc.data(), (int64_t)mc.size(), d_rad);
     if (rc) return release(rc);
@@ -1349,10 +1404,18 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
       feat[i * kFeat + 4] = 1.0f - g.top_absfrac;
     }
     const size_t pred_bytes = (size_t)n_audit * kTarget * sizeof(float);
-    if (hipMalloc(&d_feat, feat.size() * sizeof(float)) != hipSuccess ||
-        hipMalloc(&d_pred, pred_bytes) != hipSuccess ||
-        hipHostMalloc(&h_pred, pred_bytes, hipHostMallocDefault) != hipSuccess)
+    if (!reserve(p, kBufFeat, feat.size() * sizeof(float), &d_feat) ||
+        !reserve(p, kBufPred, pred_bytes, &d_pred))
       return release(FD2_E_NOMEM);
+    if (p->h_pred_cap < pred_bytes) {
+      if (p->h_pred) (void)hipHostFree(p->h_pred);
+      p->h_pred = nullptr;
+      p->h_pred_cap = 0;
+      if (hipHostMalloc(&p->h_pred, pred_bytes, hipHostMallocDefault) != hipSuccess)
+        return release(FD2_E_NOMEM);
+      p->h_pred_cap = pred_bytes;
+    }
+    h_pred = p->h_pred;
     if (hipMemcpyAsync(d_feat, feat.data(), feat.size() * sizeof(float), hipMemcpyHostToDevice,
                        p->stream) != hipSuccess)
       return release(FD2_E_DEVICE);
@@ -1381,14 +1444,14 @@ static int surrogate_submit(fd2_ctx* c, fd2_mlp* p, const fd2_event* events, int
     const int chunk = 65536;
     rc = ensure_batch(p, chunk);
     if (rc) return release(rc);
-    if (hipMalloc(&d_all, (size_t)n * sizeof(fd2_event)) != hipSuccess ||
-        hipMalloc(&d_ev, (size_t)ns * sizeof(fd2_event)) != hipSuccess ||
-        hipMalloc(&d_idx, (size_t)ns * 4) != hipSuccess ||
-        hipMalloc(&d_ids, (size_t)ns * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&d_logits, (size_t)ns * kOutPad * sizeof(float)) != hipSuccess)
+    if (!reserve(p, kBufAll, (size_t)n * sizeof(fd2_event), &d_all) ||
+        !reserve(p, kBufEv, (size_t)ns * sizeof(fd2_event), &d_ev) ||
+        !reserve(p, kBufIdx, (size_t)ns * 4, &d_idx) ||
+        !reserve(p, kBufIds, (size_t)ns * sizeof(uint64_t), &d_ids) ||
+        !reserve(p, kBufLogits, (size_t)ns * kOutPad * sizeof(float), &d_logits))
       return release(FD2_E_NOMEM);
     const size_t npix0 = (size_t)c->hgeom[0].xdim * c->hgeom[0].ydim;
-    if (hipMalloc(&d_spread, npix0 * fd2mlp::kSpreadCopies * sizeof(uint32_t)) != hipSuccess)
+    if (!reserve(p, kBufSpread, npix0 * fd2mlp::kSpreadCopies * sizeof(uint32_t), &d_spread))
       return release(FD2_E_NOMEM);
     const fd2::GeomDev& g0 = c->hgeom[0];
     const int dev_cu = c->n_cu > 0 ? c->n_cu : 1;
