@@ -268,6 +268,38 @@ def test_surrogate_path_is_deterministic(fd2lib):
         net.close()
 
 
+def test_surrogate_buffers_kept_between_calls(fd2lib):
+    """The network keeps its call buffers (events, indices, logits, PRF
+    copies, radial tallies, audit predictions) between calls and grows them
+    on demand: calls of growing and shrinking size on one network give the
+    tallies and audit statistics a fresh network gives for each."""
+    from hybridmantis_amd import events as E, surrogate as S
+
+    p, ev = E.workload("mono50", scale=0.004)
+    sizes = [len(ev) // 8, len(ev), len(ev) // 3, len(ev)]
+    keys = ("n_mc", "n_surrogate", "n_audit", "photons_mc", "audit_detected_mc",
+            "audit_detected_pred", "audit_radial_l1")
+    kept = S.load()
+    try:
+        for n in sizes:
+            fresh = S.load()
+            try:
+                with fd2lib.Transport(p) as tr:
+                    sf = fd2lib.surrogate_submit(tr, fresh, ev[:n], audit_fraction=0.05)
+                    want = _tallies(tr)
+            finally:
+                fresh.close()
+            with fd2lib.Transport(p) as tr:
+                sk = fd2lib.surrogate_submit(tr, kept, ev[:n], audit_fraction=0.05)
+                got = _tallies(tr)
+            print(f"[surrogate-buffers] {n} events", flush=True)
+            _same(got, want)
+            for k in keys:
+                assert sk[k] == sf[k], (n, k)
+    finally:
+        kept.close()
+
+
 def test_surrogate_routing_reports_the_first_invalid_event(fd2lib):
     """Routing runs over ranges of the events on several host threads; the
     status is still that of the first invalid event in order, and nothing
