@@ -226,7 +226,7 @@ def main() -> int:
     ap.add_argument("--cpu-workers", type=int, default=0)
     ap.add_argument("--cpu-photons", type=int, default=150000,
                     help="photons per CPU process in the baseline sample")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched (config 2) leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched 4-geometry leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -117,7 +117,7 @@ def test_autotuner(fd2lib, batch):
 
 
 def test_batched_launch_matches_oracle_per_photon(fd2lib, oracle):
-    """A slice of the config-2 workload (250 histories per geometry, all
+    """A slice of the batched 4-geometry workload (250 histories per geometry, all
     four geometries, events above 2,000 photons) in one multi-geometry
     launch: each geometry's photons, fates and pixels, against the oracle's
     device arithmetic on the production streams with that geometry's event
@@ -151,7 +151,7 @@ def test_batched_launch_matches_oracle_per_photon(fd2lib, oracle):
 
 
 def test_full_batched_workload_properties(fd2lib):
-    """The full config-2 workload (4 x 50,000 histories, ~4.6e8 photons):
+    """The full batched 4-geometry workload (4 x 50,000 histories, ~4.6e8 photons):
     per geometry every photon ends exactly once, the per-history counts add
     up to the detections, the PRF holds at most the detections; a graph
     replay equals a direct submit and an identical rerun."""

@@ -37,7 +37,7 @@ def test_fused_forward_equals_layered_forward(fd2lib, n):
     k order, same epilogue arithmetic.  Row counts around the 64-row
     workgroup tile and past the 65,536-row host batch; trained-like
     parameters (non-zero biases everywhere); the fused path rebuilds its
-    transposed weights after set_params and after training steps."""
+    fragment-ordered weights after set_params and after training steps."""
     rng = np.random.default_rng(n)
     p = M.init_params(11)
     for l in range(4):
@@ -51,7 +51,7 @@ def test_fused_forward_equals_layered_forward(fd2lib, n):
         # the reference at FP32-accumulation tolerance, through the probabilities
         if n <= 4097:
             assert np.abs(net.forward(x) - M.probs(p, x)).max() < 2e-3
-        if n == 300:  # after Adam steps the transposed copy is rebuilt
+        if n == 300:  # after Adam steps the fragment-ordered copy is rebuilt
             xs, ys = batch(rng, 1024)
             net.set_data(xs, ys)
             net.train(3, batch=256, lr=1e-2)
@@ -173,9 +173,9 @@ def test_surrogate_pipeline_small(fd2lib):
 
 def test_trained_surrogate_gain(fd2lib):
     """The shipped network (hybridmantis_amd/data/surrogate_mlp.npz) against
-    the method without it, at the config's own size: 1e6 histories at 50 keV
+    the method without it, at the workload's full size: 1e6 histories at 50 keV
     on a 500-um layer, surrogate + MC fallback vs the plain MC on the same
-    events, at north_star's bars: at least 10x the plain MC's histories/s,
+    events, at the accuracy and speed bars: at least 10x the plain MC's histories/s,
     PRF L1 <= 2% at the 1-um pitch (against the mean of 4 plain-MC runs of
     the same events, since two single runs already differ by ~2% there) and
     within 1.25x of what a plain-MC run of the same size scores against that

@@ -74,7 +74,7 @@ def load_reference_fixture(golden, name):
 
 @pytest.mark.parametrize("name", CASES)
 def test_trajectory_faithful_matches_reference(fd2lib, oracle, golden, name):
-    """north_star's trajectory test against the reference itself: same
+    """The trajectory test against the reference itself: same
     RANECU streams, the reference's arithmetic; fates and pixels of every
     photon against what the reference's compiled C branch produced."""
     p, ev, seed = load_case(golden, name)
@@ -110,7 +110,7 @@ def test_trajectory_faithful_full_reference_case(fd2lib, oracle):
     rounding boundary, measured at 2 events in 2.2e6 photons on the CPU."""
     p, ev = E.workload("mono25", scale=1.0)
     seed = 123456789
-    print(f"[trajectory-faithful] mono25 config 0: {len(ev)} events "
+    print(f"[trajectory-faithful] mono25 2,000 histories: {len(ev)} events "
           f"{int(ev['N'].sum())} photons", flush=True)
     with fd2lib.Transport(p) as tr:
         rec = tr.trace_ranecu(ev, seed_input=seed, faithful=True)
