@@ -95,10 +95,14 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0,
   }
 }
 
-// Production stream: xoshiro128** (Blackman & Vigna 2018) per photon, its
+// Production stream: xoshiro128+ (Blackman & Vigna 2018) per photon, its
 // 128-bit state = Philox4x32-10(counter (photon, event >> 32, 0, 0),
 // key (seed, event mod 2^32)): distinct for all 2^64 event numbers, and the
-// same as a 32-bit event key below 2^32.
+// same as a 32-bit event key below 2^32.  The + scrambler (the sum of two
+// state words) is the generator's authors' choice for floating-point
+// output: its weak low bits never reach u(), which keeps the high 24.  The
+// ** scrambler's two integer multiplies per draw cost 2% of the transport's
+// time (mono25x50: 134.7 -> 132.0 ms per launch).
 struct CounterRng {
   uint32_t s0, s1, s2, s3;
   __device__ __forceinline__ void seed(uint32_t key0, uint64_t event,
@@ -109,9 +113,8 @@ struct CounterRng {
     if ((s0 | s1 | s2 | s3) == 0u) s0 = 0x9E3779B9u;
   }
   __device__ __forceinline__ uint32_t next() {
-    uint32_t x = s1 * 5u;
-    uint32_t out = ((x << 7) | (x >> 25)) * 9u;
-    uint32_t t = s1 << 9;
+    const uint32_t out = s0 + s3;
+    const uint32_t t = s1 << 9;
     s2 ^= s0;
     s3 ^= s1;
     s1 ^= s2;

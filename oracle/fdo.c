@@ -107,7 +107,7 @@ typedef struct {
   int32_t ranecu[2];
   uint32_t key[2];
   uint32_t event_hi;  /* counter mode: event number >> 32 (counter word 1) */
-  uint32_t xs[4];     /* per-photon xoshiro128** state */
+  uint32_t xs[4];     /* per-photon xoshiro128+ state */
 
   const fdo_event *ev;
   float pos[3], dcos[3], normal[3];
@@ -122,11 +122,10 @@ typedef struct {
 static float uni(ctx *c) {
   c->t->work[FDO_W_DRAW]++;
   if (c->rng_mode == FDO_RNG_RANECU) return fdo_ranecu(c->ranecu);
-  /* xoshiro128** (Blackman & Vigna, "Scrambled linear pseudorandom number
+  /* xoshiro128+ (Blackman & Vigna, "Scrambled linear pseudorandom number
    * generators", 2018), 24 high bits -> (0,1], exact in float */
   uint32_t *s = c->xs;
-  uint32_t x = s[1] * 5u;
-  uint32_t out = ((x << 7) | (x >> 25)) * 9u;
+  uint32_t out = s[0] + s[3];
   uint32_t t = s[1] << 9;
   s[2] ^= s[0];
   s[3] ^= s[1];
@@ -137,8 +136,9 @@ static float uni(ctx *c) {
   return (float)((out >> 8) + 1u) * (1.0f / 16777216.0f);
 }
 
-/* counter mode: photon k of event e draws from a xoshiro128** stream whose
- * 128-bit state is Philox4x32-10(counter = (k, 0, 0, 0), key = (seed, e)). */
+/* counter mode: photon k of event e draws from a xoshiro128+ stream whose
+ * 128-bit state is Philox4x32-10(counter = (k, e >> 32, 0, 0), key = (seed,
+ * e mod 2^32)). */
 static void counter_photon(ctx *c, uint32_t photon) {
   uint32_t ctr[4] = {photon, c->event_hi, 0u, 0u};
   fdo_philox4x32_10(ctr, c->key, c->xs);

@@ -16,7 +16,7 @@
  *   FDO_RNG_RANECU  one RANECU stream per event, seeded like the C driver
  *                   (init_PRNG(event_index, 50000, seed_input)); the photons
  *                   of an event consume it in order.
- *   FDO_RNG_COUNTER one stream per photon: xoshiro128** whose 128-bit
+ *   FDO_RNG_COUNTER one stream per photon: xoshiro128+ whose 128-bit
  *                   state is Philox4x32-10(counter = (photon, event id
  *                   >> 32, 0, 0), key = (seed, event id mod 2^32)) -- the
  *                   production kernel's stream layout, independent of

@@ -89,7 +89,7 @@ def test_counter_streams_uniform_and_independent(oracle):
 
 
 def _xoshiro_u(state, n):
-    """xoshiro128** (Blackman & Vigna 2018) from a 4-word state, 24 high
+    """xoshiro128+ (Blackman & Vigna 2018) from a 4-word state, 24 high
     bits -> (0, 1], in Python integers."""
     s = [int(v) for v in state]
     if not any(s):
@@ -97,8 +97,7 @@ def _xoshiro_u(state, n):
     m = 0xFFFFFFFF
     out = []
     for _ in range(n):
-        x = (s[1] * 5) & m
-        r = ((((x << 7) | (x >> 25)) & m) * 9) & m
+        r = (s[0] + s[3]) & m
         t = (s[1] << 9) & m
         s[2] ^= s[0]
         s[3] ^= s[1]
@@ -109,6 +108,17 @@ def _xoshiro_u(state, n):
         out.append(np.float32(((r >> 8) + 1) / 16777216.0))
     return np.array(out, np.float32)
 
+
+
+def test_xoshiro128_plus_worked_example():
+    """Three draws from the state (1, 2, 3, 4), worked by hand from the
+    generator's definition: output s0 + s3, then t = s1 << 9, s2 ^= s0,
+    s3 ^= s1, s1 ^= s2, s0 ^= s3, s2 ^= t, s3 = rotl(s3, 11).
+    Outputs 5, 7 + 12288 = 12295 and 12295 + 25165824 = 25178119, i.e. the
+    uniforms (0 + 1), (48 + 1) and (98352 + 1) times 2^-24."""
+    got = _xoshiro_u([1, 2, 3, 4], 3)
+    want = np.array([1, 49, 98353], np.float64) / 2**24
+    assert np.array_equal(got.astype(np.float64), want)
 
 @pytest.mark.parametrize("event", [0, 5, 2**32 - 1, 2**32, 2**32 + 5, 3 * 2**32 + 77, 2**63 + 1])
 def test_counter_stream_key_is_64_bit(oracle, event):
