@@ -1134,8 +1134,11 @@ k_surr_sample(SampleArgs a, const fd2_event* __restrict__ events,
       float sn, cs;
       fd2_sincos(rng.u() * 6.283185307f, &sn, &cs);
       const float x = ev.x + r * cs, y = ev.y + r * sn;
-      const int pi = (int)floorf((x - g.lbx) / (float)g.pixel);
-      const int pj = (int)floorf((y - g.lby) / (float)g.pixel);
+      // x / pitch, as a product when the pitch is a power of two (exact)
+      const float qx = g.inv_pixel != 0.0f ? (x - g.lbx) * g.inv_pixel : (x - g.lbx) / (float)g.pixel;
+      const float qy = g.inv_pixel != 0.0f ? (y - g.lby) * g.inv_pixel : (y - g.lby) / (float)g.pixel;
+      const int pi = (int)floorf(qx);
+      const int pj = (int)floorf(qy);
       if (x <= g.ubx && y <= g.uby && x >= g.lbx && y >= g.lby && pi >= 0 && pi < g.xdim &&
           pj >= 0 && pj < g.ydim) {
         const int ti = pi - a.tile_i0, tj = pj - a.tile_j0;
