@@ -513,7 +513,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
     // run with one or two lanes if done whenever a photon ends: wait until
     // refill_lanes lanes are free, or no lane is left to step.
     unsigned long long need = __ballot(!alive);
-    if (__popcll(need) < a.refill_lanes && need != ~0ull) need = 0ull;
+    if ((int)__popcll(need) < a.refill_lanes && need != ~0ull) need = 0ull;
     if (QUEUE && need != 0ull && n_back != 0u) {
       __threadfence_block();  // the wave's queue stores are complete before its loads
       const uint32_t take = min((uint32_t)__popcll(need), n_back);
@@ -753,7 +753,7 @@ k_transport(TransportArgs a, const GeomDev* __restrict__ geoms_in,
       // it, or when nothing else can run.
       const unsigned long long parked_mask = __ballot(parked);
       if (parked_mask != 0ull &&
-          (__popcll(parked_mask) >= a.park_lanes || __ballot(alive && !parked) == 0ull)) {
+          ((int)__popcll(parked_mask) >= a.park_lanes || __ballot(alive && !parked) == 0ull)) {
         if (parked) {
           const GeomDev& g = geoms[MULTI ? gi : 0u];
           ended = ba_prpt(rng, p, g);
