@@ -100,17 +100,19 @@ def test_trajectory_faithful_matches_reference(fd2lib, oracle, golden, name):
         assert np.array_equal(rec[f], o.photons[f]), f
 
 
-def test_trajectory_faithful_full_reference_case(fd2lib, oracle):
+@pytest.mark.parametrize("scale", [1.0, 10.0])
+def test_trajectory_faithful_full_reference_case(fd2lib, oracle, scale):
     """The reference's CPU-runnable case in full (2,000 histories at 25 keV,
-    ~2.2e6 photons, 501x501 1-um PRF) on RANECU streams: the faithful GPU
-    kernel against the oracle's faithful mode (the reference's float/double
-    arithmetic with the C library, bit-exact against the compiled reference
-    on the fixtures), photon by photon.  First-divergence bar 0.1%; the
-    float-pair evaluation rounds like double except within ~2^-46 of a
-    rounding boundary, measured at 2 events in 2.2e6 photons on the CPU."""
-    p, ev = E.workload("mono25", scale=1.0)
+    ~2.2e6 photons, 501x501 1-um PRF), and ten times it, on RANECU streams:
+    the faithful GPU kernel against the oracle's faithful mode (the
+    reference's float/double arithmetic with the C library, bit-exact
+    against the compiled reference on the fixtures), photon by photon.
+    First-divergence bar 0.1%; the float-pair evaluation rounds like double
+    except within ~2^-46 of a rounding boundary, measured at 2 events in
+    2.2e6 photons on the CPU."""
+    p, ev = E.workload("mono25", scale=scale)
     seed = 123456789
-    print(f"[trajectory-faithful] mono25 2,000 histories: {len(ev)} events "
+    print(f"[trajectory-faithful] mono25 {p['num_hist']} histories: {len(ev)} events "
           f"{int(ev['N'].sum())} photons", flush=True)
     with fd2lib.Transport(p) as tr:
         rec = tr.trace_ranecu(ev, seed_input=seed, faithful=True)
