@@ -170,6 +170,26 @@ def test_production_kernel_matches_oracle_per_photon(fd2lib, oracle, golden, nam
     assert np.array_equal(rec["x"][det], o.photons["x"][det])
 
 
+@pytest.mark.parametrize("scale", [1.0, 10.0])
+def test_plain_submit_tallies_match_oracle_full_reference_case(fd2lib, oracle, scale):
+    """The launch the bench times (plain submit: no records, no radial
+    tallies, wave queues, default tuning) on the reference's CPU-runnable
+    case in full (2,000 histories at 25 keV, ~2.2e6 photons) and on ten
+    times it: PRF, per-history counts and counters equal the oracle's device
+    arithmetic on the same streams, bit for bit."""
+    p, ev = E.workload("mono25", scale=scale)
+    print(f"[plain-tallies] {len(ev)} events {int(ev['N'].sum())} photons", flush=True)
+    with fd2lib.Transport(p) as tr:
+        tr.submit(ev)
+        tr.sync()
+        prf, hist, counters = tr.prf(), tr.hist_counts(), tr.counters()
+    o, _ = oracle.run_parallel(p, ev, p["num_hist"], workers=min(16, os.cpu_count() or 1),
+                               rng=oracle.RNG_COUNTER, seed=271828182, math=oracle.MATH_DEVICE)
+    assert [counters[k] for k in oracle.COUNTER_NAMES] == o.counters.tolist()
+    assert np.array_equal(hist, o.hist_count)
+    assert np.array_equal(prf, o.prf)
+
+
 @pytest.mark.parametrize("scale", [0.25, 1.0])
 def test_statistics_match_faithful_oracle(fd2lib, oracle, scale):
     """500 and 2,000 histories at 25 keV (~5.5e5 and ~2.2e6 photons, the
