@@ -150,6 +150,34 @@ def test_batched_launch_matches_oracle_per_photon(fd2lib, oracle):
         assert [cnt[k] for k in oracle.COUNTER_NAMES] == o.counters.tolist(), g
 
 
+def test_batched_plain_submit_tallies_match_oracle(fd2lib, oracle):
+    """The multi-geometry launch without records (the path the batched
+    bench replays) on 5% of the batched workload (2,500 histories per
+    geometry, 2.3e7 photons): each geometry's PRF, per-history counts and
+    counters equal the oracle's device arithmetic on the production streams
+    with that geometry's event numbers in the batch, bit for bit."""
+    import os
+
+    plist, ev = E.batched_workload(scale=0.05)
+    print(f"[batched-plain] {len(ev)} events {int(ev['N'].sum())} photons", flush=True)
+    with fd2lib.Transport(plist) as tr:
+        tr.submit(ev)
+        tr.sync()
+        got = tallies(tr, len(plist))
+    for g, p in enumerate(plist):
+        idx = np.flatnonzero(ev["geom"] == g)
+        lo, hi = int(idx[0]), int(idx[-1]) + 1
+        assert hi - lo == len(idx)
+        o, _ = oracle.run_parallel(p, ev[lo:hi], p["num_hist"],
+                                   workers=min(16, os.cpu_count() or 1),
+                                   rng=oracle.RNG_COUNTER, seed=271828182,
+                                   math=oracle.MATH_DEVICE, event_base=lo)
+        prf, hist, cnt = got[g]
+        assert [cnt[k] for k in oracle.COUNTER_NAMES] == o.counters.tolist(), g
+        assert np.array_equal(hist, o.hist_count), g
+        assert np.array_equal(prf, o.prf), g
+
+
 def test_full_batched_workload_properties(fd2lib):
     """The full batched 4-geometry workload (4 x 50,000 histories, ~4.6e8 photons):
     per geometry every photon ends exactly once, the per-history counts add
