@@ -385,3 +385,40 @@ def test_full_workload_properties(fd2lib):
     # the PRF peaks at the beam pixel (500.5 um -> pixel 250)
     i, j = np.unravel_index(np.argmax(prf), prf.shape)
     assert abs(i - 250) <= 2 and abs(j - 250) <= 2
+
+
+def test_reference_maximum_event_buffer(fd2lib):
+    """The reference's largest event buffer, maxvalopt = 2,304,000 events
+    (main/kernel_cuda_c_ver1_0.cu:93), in one submit: 2,304,000 events of
+    0 to 100 photons (1.04e8 photons, about 10% of the events empty) over
+    20,000 histories.  Conservation of photons and detections, and the same
+    tallies as the events split over two submits (bit for bit: streams are
+    keyed by event number)."""
+    n_ev = 2304000
+    rng = np.random.default_rng(23)
+    p = E.params(num_hist=20000)
+    ev = np.zeros(n_ev, E.EVENT_DTYPE)
+    ev["x"] = 500.5 + rng.normal(0.0, 3.0, n_ev)
+    ev["y"] = 500.5 + rng.normal(0.0, 3.0, n_ev)
+    ev["z"] = rng.uniform(-0.5 * p["H"], 0.5 * p["H"], n_ev)
+    ev["E"] = 1000.0
+    ev["hist"] = np.sort(rng.integers(1, 20001, n_ev))
+    ev["N"] = rng.integers(0, 101, n_ev)
+    ev["N"][rng.random(n_ev) < 0.1] = 0
+    n = int(ev["N"].sum())
+    print(f"[max-buffer] {n_ev} events {n} photons", flush=True)
+    with fd2lib.Transport(p) as tr:
+        tr.submit(ev)
+        a = (tr.prf(), tr.hist_counts(), tr.counters())
+        tr.reset()
+        half = n_ev // 2 + 7
+        tr.submit(ev[:half])
+        tr.submit(ev[half:])
+        b = (tr.prf(), tr.hist_counts(), tr.counters())
+    c = a[2]
+    assert c["generated"] == n
+    assert n == sum(c[k] for k in ("detected", "abs_top", "abs_bulk", "lost",
+                                   "outofcol", "theta1"))
+    assert int(a[1].sum()) == c["detected"]
+    assert int(a[0].sum()) <= c["detected"]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
