@@ -131,6 +131,32 @@ def test_trajectory_faithful_full_reference_case(fd2lib, oracle, scale):
         assert np.array_equal(prf, o.prf)
 
 
+@pytest.mark.parametrize("geom", [0, 1, 2, 3])
+def test_trajectory_faithful_batched_geometries(fd2lib, oracle, geom):
+    """Each geometry of the batched workload ((H, R) = (100, 2.55),
+    (150, 2.55), (300, 4.0), (500, 5.0) um, 15-90 keV), 500 histories, on
+    RANECU streams: the faithful GPU kernel against the oracle's faithful
+    mode, photon by photon, first-divergence bar 0.1%."""
+    plist, ev = E.batched_workload(scale=0.01)
+    p = plist[geom]
+    sel = ev[ev["geom"] == geom].copy()
+    sel["geom"] = 0
+    seed = 987654321
+    print(f"[trajectory-faithful] geometry {geom}: {len(sel)} events "
+          f"{int(sel['N'].sum())} photons", flush=True)
+    with fd2lib.Transport(p) as tr:
+        rec = tr.trace_ranecu(sel, seed_input=seed, faithful=True)
+        prf = tr.prf()
+    o, _ = oracle.run_parallel(p, sel, p["num_hist"], workers=min(16, os.cpu_count() or 1),
+                               rng=oracle.RNG_RANECU, seed=seed, math=oracle.MATH_FAITHFUL,
+                               record=True)
+    bad, compared = first_divergences(sel, rec, o.photons)
+    print(f"  events diverging {bad}, photons compared {compared}", flush=True)
+    assert bad <= 0.001 * compared, (bad, compared)
+    if bad == 0:
+        assert np.array_equal(prf, o.prf)
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_trajectory_ranecu_matches_oracle(fd2lib, oracle, golden, name):
     p, ev, seed = load_case(golden, name)
